@@ -1,0 +1,139 @@
+"""Benchmark: batched H·psi on the 221,184-dimensional FAD-TrpH pair state (one MI355X).
+
+A step is one batched H·psi launch over B resident complex128 vectors of the
+production operator (theta = 90 deg, 50 uT, Efimova-Hore D/J = -3, 14N
+quadrupoles on).  value = H·psi per second over the timed steps; the roofline
+object prices the kernel's algorithmic bytes (read psi + write H psi,
+32 B per amplitude = 7,077,888 B per vector) against the 8 TB/s HBM peak.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = ("H·ψ/s at dim 221184 (% HBM peak); batched yields/s; yield+grad vs yield time")
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_AMPLITUDE = 32
+
+
+def cpu_baseline(seconds: float = 12.0) -> dict:
+    """Single-core H·psi/s of the oracle (NumPy/SciPy restatement of the reference matvec)."""
+    code = r"""
+import json, sys, time
+sys.path.insert(0, %r)
+import numpy as np
+from oracle import core
+h = core.build_hamiltonian(90.0, 0.05)
+rng = np.random.default_rng(0)
+v = rng.normal(size=h.dimension) + 1j * rng.normal(size=h.dimension)
+v /= np.linalg.norm(v)
+h.matvec(v)
+n, t0 = 0, time.perf_counter()
+while time.perf_counter() - t0 < %f:
+    v = h.matvec(v); v /= np.linalg.norm(v); n += 1
+dt = time.perf_counter() - t0
+print(json.dumps({"n": n, "dt": dt}))
+""" % (ROOT, seconds)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               NUMEXPR_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, check=True)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"value": r["n"] / r["dt"], "unit": "H·ψ/s", "cores": 1, "kind": "port",
+            "sample": f"{r['n']} sequential oracle CompiledFullPairHamiltonian.matvec calls on one vector "
+                      f"(production operator, dim 221184) over {r['dt']:.1f} s, 1 thread"}
+
+
+def measured_traffic(batch: int):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if present."""
+    path = os.path.join(ROOT, "profiles", "hpsi_pair_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    per_vector = data.get("hbm_bytes_per_vector")
+    return None if per_vector is None else per_vector * batch
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(trajectories shard independently across GPUs; see DESIGN.md).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from rpm_amd import core, workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        return 2
+    h = core.build_hamiltonian(90.0, 0.05)
+    assert h.uses_pair_stencil
+    n = h.dimension
+    B = args.batch
+    print(f"[bench] building {B} synthetic vectors of dim {n}", file=sys.stderr, flush=True)
+    x = torch.from_numpy(workloads.normal_vectors(n, B, seed=0)).cuda()
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        h.apply_device(x, y, batch=B, ld=n)
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    start.record()
+    for _ in range(args.steps):
+        h.apply_device(x, y, batch=B, ld=n)
+    stop.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = start.elapsed_time(stop)
+    ms_per_step = ms_total / args.steps
+    value = B * args.steps / (ms_total / 1e3)
+    algo_bytes = BYTES_PER_AMPLITUDE * n * B
+    achieved = algo_bytes / (ms_per_step / 1e3) / 1e9
+    print(f"[bench] {args.steps} steps in {ms_total:.3f} ms (wall {wall:.3f} s)", file=sys.stderr, flush=True)
+    result = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "H·ψ/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": "batched H·ψ, production FAD-TrpH operator (theta=90 deg, 50 uT, "
+                               "EH D/J=-3, Q on), dim 221184, complex128",
+                   "batch_vectors": B, "dimension": n, "kernel": "k_pair_apply"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B)},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline()
+        result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

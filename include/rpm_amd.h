@@ -1,0 +1,157 @@
+/*
+ * rpm_amd.h -- C ABI of the MI355X radical-pair solver library (librpm_amd.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are interleaved FP64
+ * (re, im) pairs ("c128").  Pointers named d_* are device pointers that the
+ * caller owns (the library never frees or retains them); streams are
+ * hipStream_t passed as void*.  Every function returns 0 on success or a
+ * negative RPM_E_* code; rpm_last_error() returns the message of the most
+ * recent failure on the calling thread.
+ *
+ * Reference interfaces each entry point replaces (R/ = "Realistic system
+ * size/code/" of sam-not-smart/RPM):
+ *   rpm_operator_create_pair   CompiledFullPairHamiltonian.__init__   R/rpm_statevector_engine.py:142-219
+ *   rpm_operator_create_local  MatrixFreeHamiltonian.__init__         R/rpm_statevector_engine.py:49-93
+ *   rpm_operator_apply         .matvec / .matmat                       R/rpm_statevector_engine.py:95-114,221-253
+ *   rpm_lanczos_basis          AdaptiveLanczosPropagator._basis        R/rpm_statevector_engine.py:651-688
+ *   rpm_krylov_candidates      AdaptiveLanczosPropagator._candidate    R/rpm_statevector_engine.py:690-715
+ *                              and _projected_coefficients             R/rpm_statevector_engine.py:569-582
+ *   rpm_singlet_gram           SingletPopulationObservable.project_basis R/rpm_statevector_engine.py:521-529
+ *   rpm_krylov_outputs         the per-output-time loop of propagate   R/rpm_statevector_engine.py:861-902
+ *                              with evaluate_projected                 R/rpm_statevector_engine.py:531-538
+ *   rpm_krylov_combine         endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
+ *   rpm_norm                   np.linalg.norm(state)                   R/rpm_statevector_engine.py:656,774,909
+ *   rpm_singlet_population     singlet_population                      R/rpm_statevector_engine.py:492-497
+ */
+#ifndef RPM_AMD_H
+#define RPM_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RPM_OK 0
+#define RPM_E_INVALID (-1)   /* bad argument: maps to ValueError */
+#define RPM_E_HIP (-2)       /* HIP runtime error */
+#define RPM_E_NOMEM (-3)     /* device allocation failed */
+#define RPM_E_UNSUPPORTED (-4)
+
+/* Message of the last failure on this thread (NUL-terminated, truncated to len). */
+int rpm_last_error(char* buffer, size_t len);
+
+/* Library version string. */
+const char* rpm_version(void);
+
+typedef struct rpm_operator rpm_operator;
+
+/*
+ * Two-radical compiled Hamiltonian.  State layout (C order):
+ *   index = ((eA * 2 + eB) * NL + l) * NR + r
+ * with l the C-order multi-index over the left (radical A) nuclear axes and r
+ * over the right (radical B) nuclear axes; NL = prod(left_dims), NR = prod(right_dims).
+ * nuclear_blocks: for each nucleus (left ones first, then right ones), the
+ *   (2d x 2d) complex matrix of every term acting on (its radical's electron,
+ *   that nucleus) or on the nucleus alone, row-major in the basis
+ *   (electron m_e, nuclear m): row = e * d + m.  Packed back to back.
+ * electron_pair: 4x4 complex on (eA, eB), row = eA*2 + eB; it holds the
+ *   electron-pair term and the single-electron terms (Zeeman).
+ */
+int rpm_operator_create_pair(int n_left, const int* left_dims, int n_right, const int* right_dims,
+                             const double* nuclear_blocks, const double* electron_pair,
+                             rpm_operator** out);
+
+/*
+ * Generic sum of local terms on a tensor-product state (any axis order).
+ * term_naxes[t] axes per term, term_axes packed, term_ops packed complex
+ * (D_t x D_t row-major with D_t = prod of the term's axis dimensions,
+ * C order over the term's axes as listed).
+ */
+int rpm_operator_create_local(int naxes, const int* dims, int nterms, const int* term_naxes,
+                              const int* term_axes, const double* term_ops, rpm_operator** out);
+
+int rpm_operator_destroy(rpm_operator* op);
+
+/* Dimension N of the operator's state space (or -1 for a NULL handle). */
+long long rpm_operator_dimension(const rpm_operator* op);
+
+/*
+ * d_out[b] = H d_in[b] for b < batch; vectors are ld complex elements apart.
+ * d_in and d_out must not overlap.
+ */
+int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, int batch,
+                       long long ld, void* stream);
+
+/* Bytes of device workspace rpm_lanczos_basis needs for max dimension m. */
+size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m);
+
+/*
+ * CGS2 Lanczos basis of d_state (not modified), up to `target` vectors.
+ * d_basis: target * N complex (vector j at offset j*N).
+ * d_alpha[target], d_beta[target] (FP64, device), d_info[4] (int, device):
+ *   info[0] = basis size actually built (< target on breakdown),
+ * d_norm (device double): receives ||state||.
+ * Matvec count on the host side equals info[0] (one per basis vector).
+ */
+int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, void* d_basis,
+                      double* d_alpha, double* d_beta, int* d_info, double* d_norm,
+                      double breakdown_tolerance, void* d_workspace, size_t workspace_bytes,
+                      void* stream);
+
+/*
+ * Embedded error estimates for a ladder of candidate dimensions.
+ * For each candidate i: dim = min(dims[i], size); low = max(1, dim - offset);
+ *   err[i] = safety * state_norm * || c_dim(t) - pad(c_low(t)) ||  (0 if low == dim)
+ * with c_k(t) = exp(-i T_k t) e_1 from the device tridiagonal eigensolver.
+ * d_coeffs: ncand * m_stride complex, c_dim(t) of each candidate (padded with 0).
+ * Host arrays: dims[ncand]; d_alpha/d_beta/d_size/d_norm are device pointers.
+ * d_err: device double[ncand].
+ */
+int rpm_krylov_candidates(const double* d_alpha, const double* d_beta, const int* d_info,
+                          const double* d_norm, int ncand, const int* dims, int offset,
+                          double safety, double t, int m_stride, double* d_err, void* d_coeffs,
+                          void* d_scratch, void* stream);
+
+/* Bytes of scratch rpm_krylov_candidates / rpm_krylov_outputs need for m_stride. */
+size_t rpm_krylov_scratch_bytes(int ncand, int m_stride);
+
+/* P = A^H A with A = (V[e=01] - V[e=10]) / sqrt2 over the nuclear index, Hermitized.
+ * d_basis: k vectors of length N (= 4 * nn); d_P: k*k complex row-major. */
+int rpm_singlet_gram(const void* d_basis, int k, long long nn, void* d_P, void* d_workspace,
+                     size_t workspace_bytes, void* stream);
+size_t rpm_singlet_gram_workspace_bytes(int k, long long nn);
+
+/*
+ * Values at output times inside one accepted step.
+ * sel: accepted dimension; low = max(1, sel - offset), no low model if exact or low == sel.
+ * For each offset t_i: c = c_sel(t_i); P_S = norm^2 Re(c^H P c);
+ *   point_err = safety * norm * || c - pad(c_low(t_i)) ||.
+ * low = 0 means no embedded model (point_err = 0).
+ * d_values / d_point_err: device double[nt].  d_coeffs_out (nullable): nt x sel
+ * complex, the coefficient vectors c_sel(t_i) (for point states).
+ */
+int rpm_krylov_outputs(const double* d_alpha, const double* d_beta, const double* d_norm,
+                       const void* d_P, int sel, int low, int nt, const double* d_offsets,
+                       double safety, double* d_values, double* d_point_err, void* d_coeffs_out,
+                       void* d_scratch, size_t scratch_bytes, void* stream);
+
+/* d_out = scale * sum_{j<k} d_basis[j] * c[j]; c: device complex[k]; scale: device double. */
+int rpm_krylov_combine(const void* d_basis, int k, long long n, const void* d_c,
+                       const double* d_scale, void* d_out, void* stream);
+
+/* d_out[b] = ||x_b|| for b < batch (FP64). */
+int rpm_norm(const void* d_x, long long n, int batch, long long ld, double* d_out,
+             void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* d_out[b] = singlet population of x_b (N = 4 * nn). */
+int rpm_singlet_population(const void* d_x, long long nn, int batch, long long ld, double* d_out,
+                           void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Workspace for rpm_norm / rpm_singlet_population. */
+size_t rpm_reduce_workspace_bytes(long long n, int batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RPM_AMD_H */

@@ -1,0 +1,448 @@
+// Hamiltonian action H·psi on MI355X.
+//
+// Two operator forms:
+//  * pair stencil -- the compiled two-radical form (the production H of
+//    R/rpm_statevector_engine.py:132-245).  H is rewritten as
+//      diagonal(eA,eB,l,r) + electron flips + per-nucleus off-diagonal couplings
+//    so every output amplitude is one fused gather: ~37 complex FMAs per
+//    amplitude for the 221,184-dimensional FAD-TrpH pair, no intermediate
+//    state passes (the reference makes two 16-axis transposes and two CSR
+//    products per matvec).
+//  * local terms -- any sum of LocalTerms on any axis order
+//    (R/rpm_statevector_engine.py:95-108): one gather per term per amplitude.
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "rpm_common.hpp"
+
+namespace rpm {
+
+// ---------------------------------------------------------------- pair stencil
+
+struct PairTables {
+  int NL, NR, KL, KR;
+  long long NLR;
+  const c128* dA;       // [2][NL]  diagonal from left nuclei
+  const c128* dB;       // [2][NR]
+  const c128* cA;       // [4][NL]  eA-flip coefficient, index (eA*2+eB)*NL + l
+  const c128* cB;       // [4][NR]  eB-flip coefficient, index (eA*2+eB)*NR + r
+  const int* offL;      // [NL][KL] flat offset of the k-th left neighbour ((l2-l)*NR)
+  const int* offR;      // [NR][KR] r2 - r
+  const c128* coefL;    // [NL][KL][2][2]  (eA, e')
+  const c128* coefR;    // [NR][KR][2][2]  (eB, e')
+  c128 pdiag[4];        // electron-pair diagonal
+  c128 pboth[4];        // (eA,eB) -> (1-eA,1-eB)
+};
+
+struct PairOp {
+  PairTables t;
+  void* dev;            // single device allocation backing the tables
+};
+
+__global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __restrict__ in,
+                                                    c128* __restrict__ out, long long ld,
+                                                    const double* __restrict__ in_scale,
+                                                    c128* __restrict__ copy_out,
+                                                    const int* __restrict__ active) {
+  if (active && *active == 0) return;
+  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= T.NLR) return;
+  const long long b = blockIdx.y;
+  const c128* x = in + b * ld;
+  c128* y = out + b * ld;
+  const int l = static_cast<int>(t / T.NR);
+  const int r = static_cast<int>(t - static_cast<long long>(l) * T.NR);
+  const long long S = T.NLR;
+
+  c128 p[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) p[e] = x[e * S + t];
+
+  c128 acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const int eA = e >> 1, eB = e & 1;
+    const c128 d = cadd(cadd(T.dA[eA * T.NL + l], T.dB[eB * T.NR + r]), T.pdiag[e]);
+    acc[e] = cfma(d, p[e], cmake(0.0, 0.0));
+    acc[e] = cfma(T.cA[e * T.NL + l], p[e ^ 2], acc[e]);
+    acc[e] = cfma(T.cB[e * T.NR + r], p[e ^ 1], acc[e]);
+    acc[e] = cfma(T.pboth[e], p[e ^ 3], acc[e]);
+  }
+
+  // left nuclei: couple (eA, m_n) with (e', m'), eB and r unchanged
+  for (int k = 0; k < T.KL; ++k) {
+    const long long o = t + T.offL[l * T.KL + k];
+    const c128* C = T.coefL + (static_cast<long long>(l) * T.KL + k) * 4;
+    const c128 c00 = C[0], c01 = C[1], c10 = C[2], c11 = C[3];
+    const c128 q0 = x[0 * S + o], q1 = x[1 * S + o], q2 = x[2 * S + o], q3 = x[3 * S + o];
+    // e = eA*2+eB ; source e' = e'A*2 + eB
+    acc[0] = cfma(c00, q0, acc[0]); acc[0] = cfma(c01, q2, acc[0]);
+    acc[1] = cfma(c00, q1, acc[1]); acc[1] = cfma(c01, q3, acc[1]);
+    acc[2] = cfma(c10, q0, acc[2]); acc[2] = cfma(c11, q2, acc[2]);
+    acc[3] = cfma(c10, q1, acc[3]); acc[3] = cfma(c11, q3, acc[3]);
+  }
+  // right nuclei: couple (eB, m_n) with (e', m'), eA and l unchanged
+  for (int k = 0; k < T.KR; ++k) {
+    const long long o = t + T.offR[r * T.KR + k];
+    const c128* C = T.coefR + (static_cast<long long>(r) * T.KR + k) * 4;
+    const c128 c00 = C[0], c01 = C[1], c10 = C[2], c11 = C[3];
+    const c128 q0 = x[0 * S + o], q1 = x[1 * S + o], q2 = x[2 * S + o], q3 = x[3 * S + o];
+    acc[0] = cfma(c00, q0, acc[0]); acc[0] = cfma(c01, q1, acc[0]);
+    acc[1] = cfma(c10, q0, acc[1]); acc[1] = cfma(c11, q1, acc[1]);
+    acc[2] = cfma(c00, q2, acc[2]); acc[2] = cfma(c01, q3, acc[2]);
+    acc[3] = cfma(c10, q2, acc[3]); acc[3] = cfma(c11, q3, acc[3]);
+  }
+
+  const double s = in_scale ? *in_scale : 1.0;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) y[e * S + t] = cscale(acc[e], s);
+  if (copy_out) {
+    c128* z = copy_out + b * ld;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) z[e * S + t] = cscale(p[e], s);
+  }
+}
+
+// Host-side compilation of the per-nucleus blocks into the gather tables.
+static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const double* blocks,
+                      const double* pair, PairOp** out) {
+  if (nL < 0 || nR < 0 || nL + nR > 40) return fail(RPM_E_INVALID, "bad nucleus counts");
+  long long NL = 1, NR = 1;
+  int KL = 0, KR = 0;
+  for (int i = 0; i < nL; ++i) {
+    if (ldims[i] < 1 || ldims[i] > 16) return fail(RPM_E_INVALID, "left dim %d out of range", ldims[i]);
+    NL *= ldims[i];
+    KL += ldims[i] - 1;
+  }
+  for (int i = 0; i < nR; ++i) {
+    if (rdims[i] < 1 || rdims[i] > 16) return fail(RPM_E_INVALID, "right dim %d out of range", rdims[i]);
+    NR *= rdims[i];
+    KR += rdims[i] - 1;
+  }
+  if (NL * NR * 4 > (1LL << 31)) return fail(RPM_E_INVALID, "state too large");
+  // block pointers
+  std::vector<const c128*> blk(nL + nR);
+  {
+    const c128* p = reinterpret_cast<const c128*>(blocks);
+    for (int i = 0; i < nL + nR; ++i) {
+      blk[i] = p;
+      const int d = i < nL ? ldims[i] : rdims[i - nL];
+      p += 4 * d * d;
+    }
+  }
+  const c128* P = reinterpret_cast<const c128*>(pair);
+
+  // one side's tables: digits by C-order over dims
+  auto side = [&](int n, const int* dims, long long N, int K, int first_block, long long unit,
+                  std::vector<c128>& dtab, std::vector<c128>& ftab, std::vector<int>& off,
+                  std::vector<c128>& coef) {
+    dtab.assign(2 * N, c128{0, 0});
+    ftab.assign(2 * N, c128{0, 0});   // [e][idx]: e -> 1-e, same nuclear index
+    off.assign(N * K > 0 ? N * K : 1, 0);
+    coef.assign(N * K > 0 ? N * K * 4 : 4, c128{0, 0});
+    std::vector<long long> stride(n);
+    long long s = 1;
+    for (int i = n - 1; i >= 0; --i) { stride[i] = s; s *= dims[i]; }
+    for (long long idx = 0; idx < N; ++idx) {
+      int k = 0;
+      for (int i = 0; i < n; ++i) {
+        const int d = dims[i];
+        const int m = static_cast<int>((idx / stride[i]) % d);
+        const c128* B = blk[first_block + i];       // (2d x 2d), row = e*d + m
+        const int D = 2 * d;
+        for (int e = 0; e < 2; ++e) {
+          dtab[e * N + idx] = cadd(dtab[e * N + idx], B[(e * d + m) * D + (e * d + m)]);
+          const c128 f = B[(e * d + m) * D + ((1 - e) * d + m)];
+          ftab[e * N + idx] = cadd(ftab[e * N + idx], f);
+        }
+        for (int m2 = 0; m2 < d; ++m2) {
+          if (m2 == m) continue;
+          off[idx * K + k] = static_cast<int>((m2 - m) * stride[i] * unit);
+          for (int e = 0; e < 2; ++e)
+            for (int e2 = 0; e2 < 2; ++e2)
+              coef[(idx * K + k) * 4 + e * 2 + e2] = B[(e * d + m) * D + (e2 * d + m2)];
+          ++k;
+        }
+      }
+    }
+  };
+
+  std::vector<c128> dA, dB;
+  std::vector<c128> fA, fB, coefL, coefR;
+  std::vector<int> offL, offR;
+  side(nL, ldims, NL, KL, 0, NR, dA, fA, offL, coefL);
+  side(nR, rdims, NR, KR, nL, 1, dB, fB, offR, coefR);
+
+  // combine with the electron pair term
+  auto Pel = [&](int row, int col) { return P[row * 4 + col]; };
+  std::vector<c128> cA(4 * NL), cB(4 * NR);
+  for (int e = 0; e < 4; ++e) {
+    const int eA = e >> 1, eB = e & 1;
+    for (long long l = 0; l < NL; ++l) cA[e * NL + l] = cadd(fA[eA * NL + l], Pel(e, e ^ 2));
+    for (long long r = 0; r < NR; ++r) cB[e * NR + r] = cadd(fB[eB * NR + r], Pel(e, e ^ 1));
+  }
+
+  PairOp* op = new PairOp();
+  PairTables& T = op->t;
+  T.NL = static_cast<int>(NL);
+  T.NR = static_cast<int>(NR);
+  T.KL = KL;
+  T.KR = KR;
+  T.NLR = NL * NR;
+  for (int e = 0; e < 4; ++e) {
+    T.pdiag[e] = Pel(e, e);
+    T.pboth[e] = Pel(e, e ^ 3);
+  }
+  // pack device buffer
+  size_t bytes = 0;
+  auto reserve = [&](size_t n) { size_t o = bytes; bytes += (n + 255) & ~size_t(255); return o; };
+  const size_t o_dA = reserve(dA.size() * 16), o_dB = reserve(dB.size() * 16);
+  const size_t o_cA = reserve(cA.size() * 16), o_cB = reserve(cB.size() * 16);
+  const size_t o_offL = reserve(offL.size() * 4), o_offR = reserve(offR.size() * 4);
+  const size_t o_coL = reserve(coefL.size() * 16), o_coR = reserve(coefR.size() * 16);
+  std::vector<char> host(bytes);
+  std::memcpy(host.data() + o_dA, dA.data(), dA.size() * 16);
+  std::memcpy(host.data() + o_dB, dB.data(), dB.size() * 16);
+  std::memcpy(host.data() + o_cA, cA.data(), cA.size() * 16);
+  std::memcpy(host.data() + o_cB, cB.data(), cB.size() * 16);
+  std::memcpy(host.data() + o_offL, offL.data(), offL.size() * 4);
+  std::memcpy(host.data() + o_offR, offR.data(), offR.size() * 4);
+  std::memcpy(host.data() + o_coL, coefL.data(), coefL.size() * 16);
+  std::memcpy(host.data() + o_coR, coefR.data(), coefR.size() * 16);
+  void* dev = nullptr;
+  if (hipMalloc(&dev, bytes) != hipSuccess) {
+    delete op;
+    return fail(RPM_E_NOMEM, "hipMalloc(%zu) for pair tables failed", bytes);
+  }
+  if (hipMemcpy(dev, host.data(), bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(dev);
+    delete op;
+    return fail(RPM_E_HIP, "copy of pair tables failed");
+  }
+  char* base = static_cast<char*>(dev);
+  T.dA = reinterpret_cast<const c128*>(base + o_dA);
+  T.dB = reinterpret_cast<const c128*>(base + o_dB);
+  T.cA = reinterpret_cast<const c128*>(base + o_cA);
+  T.cB = reinterpret_cast<const c128*>(base + o_cB);
+  T.offL = reinterpret_cast<const int*>(base + o_offL);
+  T.offR = reinterpret_cast<const int*>(base + o_offR);
+  T.coefL = reinterpret_cast<const c128*>(base + o_coL);
+  T.coefR = reinterpret_cast<const c128*>(base + o_coR);
+  op->dev = dev;
+  *out = op;
+  return RPM_OK;
+}
+
+// ---------------------------------------------------------------- generic local terms
+
+constexpr int kMaxTermAxes = 8;
+constexpr int kMaxLocal = 64;
+
+struct LocalTermDev {
+  int naxes, D;
+  long long stride[kMaxTermAxes];  // state strides of the term axes (as listed)
+  int dim[kMaxTermAxes];
+  int op_offset;                   // into ops (complex elements)
+  int col_offset;                  // into cols (long long elements)
+};
+
+struct LocalOp {
+  int nterms;
+  long long n;
+  LocalTermDev* terms;   // device
+  c128* ops;             // device
+  long long* cols;       // device: flat offset of local column b relative to the base
+  void* dev;
+};
+
+__global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
+                                                     const LocalTermDev* __restrict__ terms,
+                                                     const c128* __restrict__ ops,
+                                                     const long long* __restrict__ cols,
+                                                     const c128* __restrict__ in,
+                                                     c128* __restrict__ out, long long ld,
+                                                     const double* __restrict__ in_scale,
+                                                     c128* __restrict__ copy_out,
+                                                     const int* __restrict__ active) {
+  if (active && *active == 0) return;
+  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long b = blockIdx.y;
+  const c128* x = in + b * ld;
+  c128 acc{0.0, 0.0};
+  for (int t = 0; t < nterms; ++t) {
+    const LocalTermDev& T = terms[t];
+    int a = 0;
+    long long base = i;
+    for (int k = 0; k < T.naxes; ++k) {
+      const int dig = static_cast<int>((i / T.stride[k]) % T.dim[k]);
+      a = a * T.dim[k] + dig;
+      base -= dig * T.stride[k];
+    }
+    const c128* row = ops + T.op_offset + static_cast<long long>(a) * T.D;
+    const long long* cc = cols + T.col_offset;
+    for (int c = 0; c < T.D; ++c) acc = cfma(row[c], x[base + cc[c]], acc);
+  }
+  const double s = in_scale ? *in_scale : 1.0;
+  out[b * ld + i] = cscale(acc, s);
+  if (copy_out) copy_out[b * ld + i] = cscale(x[i], s);
+}
+
+static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes, const int* taxes,
+                       const double* tops, LocalOp** out) {
+  if (naxes < 1 || naxes > 64) return fail(RPM_E_INVALID, "naxes out of range");
+  std::vector<long long> stride(naxes);
+  long long n = 1;
+  for (int a = naxes - 1; a >= 0; --a) {
+    if (dims[a] < 1) return fail(RPM_E_INVALID, "Every tensor dimension must be positive.");
+    stride[a] = n;
+    n *= dims[a];
+  }
+  if (n > (1LL << 31)) return fail(RPM_E_INVALID, "state too large");
+  std::vector<LocalTermDev> terms(nterms);
+  std::vector<c128> ops;
+  std::vector<long long> cols;
+  const int* ax = taxes;
+  const c128* op = reinterpret_cast<const c128*>(tops);
+  for (int t = 0; t < nterms; ++t) {
+    LocalTermDev& T = terms[t];
+    T.naxes = tnaxes[t];
+    if (T.naxes < 1 || T.naxes > kMaxTermAxes)
+      return fail(RPM_E_INVALID, "term %d has %d axes (1..%d supported)", t, T.naxes, kMaxTermAxes);
+    int D = 1;
+    for (int k = 0; k < T.naxes; ++k) {
+      const int a = ax[k];
+      if (a < 0 || a >= naxes) return fail(RPM_E_INVALID, "term %d contains an out-of-range axis.", t);
+      for (int k2 = 0; k2 < k; ++k2)
+        if (ax[k2] == a) return fail(RPM_E_INVALID, "term %d must have unique active axes.", t);
+      T.stride[k] = stride[a];
+      T.dim[k] = dims[a];
+      D *= dims[a];
+    }
+    if (D > kMaxLocal) return fail(RPM_E_UNSUPPORTED, "term %d local dimension %d > %d", t, D, kMaxLocal);
+    T.D = D;
+    T.op_offset = static_cast<int>(ops.size());
+    T.col_offset = static_cast<int>(cols.size());
+    for (int e = 0; e < D * D; ++e) ops.push_back(op[e]);
+    for (int c = 0; c < D; ++c) {
+      long long o = 0;
+      int rem = c;
+      for (int k = T.naxes - 1; k >= 0; --k) {
+        o += static_cast<long long>(rem % T.dim[k]) * T.stride[k];
+        rem /= T.dim[k];
+      }
+      cols.push_back(o);
+    }
+    ax += T.naxes;
+    op += D * D;
+  }
+  if (ops.empty()) ops.push_back(c128{0, 0});
+  if (cols.empty()) cols.push_back(0);
+  const size_t b_terms = (terms.size() * sizeof(LocalTermDev) + 255) & ~size_t(255);
+  const size_t b_ops = (ops.size() * 16 + 255) & ~size_t(255);
+  const size_t b_cols = cols.size() * 8;
+  void* dev = nullptr;
+  if (hipMalloc(&dev, b_terms + b_ops + b_cols + 256) != hipSuccess)
+    return fail(RPM_E_NOMEM, "hipMalloc for local-term tables failed");
+  char* base = static_cast<char*>(dev);
+  bool ok = true;
+  if (!terms.empty())
+    ok = ok && hipMemcpy(base, terms.data(), terms.size() * sizeof(LocalTermDev), hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(base + b_terms, ops.data(), ops.size() * 16, hipMemcpyHostToDevice) == hipSuccess;
+  ok = ok && hipMemcpy(base + b_terms + b_ops, cols.data(), b_cols, hipMemcpyHostToDevice) == hipSuccess;
+  if (!ok) {
+    (void)hipFree(dev);
+    return fail(RPM_E_HIP, "copy of local-term tables failed");
+  }
+  LocalOp* L = new LocalOp();
+  L->nterms = nterms;
+  L->n = n;
+  L->terms = reinterpret_cast<LocalTermDev*>(base);
+  L->ops = reinterpret_cast<c128*>(base + b_terms);
+  L->cols = reinterpret_cast<long long*>(base + b_terms + b_ops);
+  L->dev = dev;
+  *out = L;
+  return RPM_OK;
+}
+
+int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int batch, long long ld,
+                      const double* in_scale, c128* copy_out, const int* active, hipStream_t stream) {
+  if (!op) return fail(RPM_E_INVALID, "NULL operator");
+  if (batch < 1 || batch > 65535) return fail(RPM_E_INVALID, "batch %d out of range", batch);
+  if (ld < op->n) return fail(RPM_E_INVALID, "ld %lld smaller than the dimension %lld", ld, op->n);
+  if (!in || !out) return fail(RPM_E_INVALID, "NULL vector");
+  if (op->kind == 1) {
+    const PairOp* P = static_cast<const PairOp*>(op->impl);
+    dim3 grid(ceil_div(P->t.NLR, 256), batch);
+    hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, in_scale,
+                       copy_out, active);
+  } else {
+    const LocalOp* L = static_cast<const LocalOp*>(op->impl);
+    dim3 grid(ceil_div(L->n, 256), batch);
+    hipLaunchKernelGGL(k_local_apply, grid, dim3(256), 0, stream, L->nterms, L->n, L->terms, L->ops,
+                       L->cols, in, out, ld, in_scale, copy_out, active);
+  }
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+}  // namespace rpm
+
+using namespace rpm;
+
+extern "C" int rpm_operator_create_pair(int n_left, const int* left_dims, int n_right,
+                                        const int* right_dims, const double* nuclear_blocks,
+                                        const double* electron_pair, rpm_operator** out) {
+  if (!out || !electron_pair || (!nuclear_blocks && n_left + n_right > 0))
+    return fail(RPM_E_INVALID, "NULL argument");
+  PairOp* P = nullptr;
+  int rc = build_pair(n_left, left_dims, n_right, right_dims, nuclear_blocks, electron_pair, &P);
+  if (rc) return rc;
+  rpm_operator* h = new rpm_operator();
+  h->kind = 1;
+  h->n = 4LL * P->t.NLR;
+  h->impl = P;
+  *out = h;
+  return RPM_OK;
+}
+
+extern "C" int rpm_operator_create_local(int naxes, const int* dims, int nterms, const int* term_naxes,
+                                         const int* term_axes, const double* term_ops,
+                                         rpm_operator** out) {
+  if (!out || !dims || (nterms > 0 && (!term_naxes || !term_axes || !term_ops)) || nterms < 0)
+    return fail(RPM_E_INVALID, "NULL argument");
+  LocalOp* L = nullptr;
+  int rc = build_local(naxes, dims, nterms, term_naxes, term_axes, term_ops, &L);
+  if (rc) return rc;
+  rpm_operator* h = new rpm_operator();
+  h->kind = 2;
+  h->n = L->n;
+  h->impl = L;
+  *out = h;
+  return RPM_OK;
+}
+
+extern "C" int rpm_operator_destroy(rpm_operator* op) {
+  if (!op) return RPM_OK;
+  if (op->kind == 1) {
+    PairOp* P = static_cast<PairOp*>(op->impl);
+    if (P->dev) (void)hipFree(P->dev);
+    delete P;
+  } else if (op->kind == 2) {
+    LocalOp* L = static_cast<LocalOp*>(op->impl);
+    if (L->dev) (void)hipFree(L->dev);
+    delete L;
+  }
+  delete op;
+  return RPM_OK;
+}
+
+extern "C" long long rpm_operator_dimension(const rpm_operator* op) { return op ? op->n : -1; }
+
+extern "C" int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, int batch,
+                                  long long ld, void* stream) {
+  if (d_in == d_out) return fail(RPM_E_INVALID, "d_in and d_out must not overlap");
+  return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld,
+                           nullptr, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}

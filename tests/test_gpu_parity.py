@@ -1,0 +1,302 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Bars: H·psi relative L2 error <= 1e-13; Lanczos coefficients / basis to
+~1e-12; propagated singlet populations and yields within 1e-9 absolute on the
+same seeds with identical step schedules.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+from oracle import core as ocore  # noqa: E402
+from oracle import dense as odense  # noqa: E402
+from oracle import engine as oengine  # noqa: E402
+from oracle import model as omodel  # noqa: E402
+from rpm_amd import core, engine, model, workloads  # noqa: E402
+
+HPSI_TOL = 1e-13
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+@pytest.fixture(scope="module")
+def production_pair():
+    h = core.build_hamiltonian(90.0, 0.05)
+    o = ocore.build_hamiltonian(90.0, 0.05)
+    return h, o
+
+
+def test_gpu_visible():
+    assert torch.cuda.is_available()
+    print("device:", torch.cuda.get_device_name(0), flush=True)
+
+
+def test_pair_stencil_full_system_matches_oracle(production_pair):
+    h, o = production_pair
+    assert h.uses_pair_stencil
+    v = workloads.normal_vectors(h.dimension, 3, seed=0)
+    for b in range(3):
+        got = h.matvec(v[b])
+        want = o.matvec(v[b])
+        assert _rel(got, want) <= HPSI_TOL
+    # independent dense-block evaluation: H psi = L M + M R^T + pair term
+    M = np.transpose(v[0].reshape(o.dimensions), o.permutation).reshape(o.left_dimension, o.right_dimension)
+    out = o.left_dense @ M + M @ o.right_dense.T
+    four = M.reshape(2, o.left_dimension // 2, 2, o.right_dimension // 2)
+    out += np.einsum("abcd,cxdy->axby", o.electron_pair_operator, four).reshape(M.shape)
+    want2 = np.transpose(out.reshape(o.left_dimensions + o.right_dimensions), o.inverse_permutation).reshape(-1)
+    assert _rel(h.matvec(v[0]), want2) <= HPSI_TOL
+
+
+def test_pair_stencil_batched_and_torch_inputs(production_pair):
+    h, o = production_pair
+    v = workloads.normal_vectors(h.dimension, 4, seed=1)
+    got = h.matmat(v.T)
+    for b in range(4):
+        assert _rel(got[:, b], o.matvec(v[b])) <= HPSI_TOL
+    t = torch.from_numpy(v[2]).cuda()
+    out = h.matvec(t)
+    assert out.is_cuda
+    assert _rel(out.cpu().numpy(), o.matvec(v[2])) <= HPSI_TOL
+
+
+@pytest.mark.parametrize("theta", [0.0, 37.0, 90.0])
+def test_pair_stencil_orientations_and_random_tensors(theta):
+    sysr = workloads.full_random_system(seed=2)
+    h = sysr.hamiltonian(theta_deg=theta)
+    o = sysr.hamiltonian(theta_deg=theta, builder=ocore.build_hamiltonian)
+    v = workloads.normal_vectors(h.dimension, 1, seed=3)[0]
+    assert _rel(h.matvec(v), o.matvec(v)) <= HPSI_TOL
+
+
+def test_generic_local_kernel_full_system():
+    h = engine.build_quantum_needle_hamiltonian(90.0, efimova_hore_D_MHz=-9.7405, exchange_J_MHz=3.2468)
+    o = oengine.build_quantum_needle_hamiltonian(90.0, efimova_hore_D_MHz=-9.7405, exchange_J_MHz=3.2468)
+    v = workloads.normal_vectors(h.dimension, 1, seed=4)[0]
+    assert _rel(h.matvec(v), o.matvec(v)) <= HPSI_TOL
+    # force the generic gather kernel on the same terms
+    g = engine._create_local(h.dimensions, h.terms)
+    x = torch.from_numpy(v).cuda()
+    y = torch.empty_like(x)
+    g.apply(x, y)
+    assert _rel(y.cpu().numpy(), o.matvec(v)) <= HPSI_TOL
+
+
+def test_small_systems_match_brute_force():
+    sx, sy, sz = model.spin_operators(0.5)
+    sy1 = model.spin_operators(1.0)[1]
+    dims = (2, 3, 2)
+    terms = (engine.LocalTerm((0, 2), np.kron(sx, sz)), engine.LocalTerm((1,), sy1))
+    h = engine.MatrixFreeHamiltonian(dims, terms)
+    expected = np.kron(sx, np.kron(np.eye(3), sz)) + np.kron(np.eye(2), np.kron(sy1, np.eye(2)))
+    np.testing.assert_allclose(h.explicit_matrix(), expected, atol=1e-13)
+    rng = np.random.default_rng(5)
+    a = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+    h4 = engine.MatrixFreeHamiltonian((2, 2, 2, 2), [engine.LocalTerm((3, 0, 2, 1), a + a.conj().T)])
+    brute = odense.dense_from_terms((2, 2, 2, 2), h4.terms)
+    np.testing.assert_allclose(h4.explicit_matrix(), brute, atol=1e-12)
+    sm = workloads.small_system(0)
+    hs = sm.hamiltonian()
+    assert hs.uses_pair_stencil
+    np.testing.assert_allclose(hs.explicit_matrix(), sm.hamiltonian(builder=ocore.build_hamiltonian).explicit_matrix(),
+                               atol=1e-11)
+
+
+def test_singlet_population_and_norm():
+    h = core.build_hamiltonian(0.0, 0.05)
+    s = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 3)
+    assert abs(engine.singlet_population(s, h.dimensions) - 1.0) < 1e-12
+    v = workloads.normal_vectors(h.dimension, 1, seed=6)[0]
+    assert abs(engine.singlet_population(v, h.dimensions) - oengine.singlet_population(v, h.dimensions)) < 1e-14
+    t = torch.from_numpy(v * 3.0).cuda()
+    assert abs(float(engine.device_norm(t).item()) - 3.0) < 1e-13
+
+
+def _small_pair(seed=3):
+    rng = np.random.default_rng(seed)
+
+    def rsym():
+        a = rng.normal(0.0, 0.3, size=(3, 3))
+        return 0.5 * (a + a.T) + rng.uniform(0.0, 1.0) * np.eye(3)
+
+    def traceless():
+        a = rng.normal(size=(3, 3))
+        a = 0.5 * (a + a.T)
+        return (a - np.trace(a) / 3 * np.eye(3)) / 28.0
+
+    nuclei = (model.Nucleus("N5", 1.0, rsym()), model.Nucleus("H6", 0.5, rsym()), model.Nucleus("N1", 1.0, rsym()))
+    radicals = ("FAD", "FAD", "TrpH")
+    q = {"FAD:N5": traceless(), "TrpH:N1": traceless()}
+    h = core.build_hamiltonian(37.0, 0.05, nuclei=nuclei, radicals=radicals, quadrupole_tensors=q)
+    o = ocore.build_hamiltonian(37.0, 0.05, nuclei=nuclei, radicals=radicals, quadrupole_tensors=q)
+    return h, o
+
+
+def test_lanczos_basis_matches_oracle_cgs2():
+    h, o = _small_pair()
+    state = core.random_phase_initial_state(h.dimensions, 11, 2)
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=8, maximum_dimension=16, embedded_offset=4)
+    oprop = oengine.AdaptiveLanczosPropagator(o, minimum_dimension=8, maximum_dimension=16, embedded_offset=4)
+    x = torch.from_numpy(state).cuda()
+    size = prop._basis(x, 16)
+    ov, oa, ob = oprop._basis(state, 16)
+    ws = prop._workspace()
+    assert size == oa.size == 16
+    np.testing.assert_allclose(ws.alpha[:size].cpu().numpy(), oa, rtol=0, atol=1e-10 * np.abs(oa).max())
+    np.testing.assert_allclose(ws.beta[: size - 1].cpu().numpy(), ob, rtol=1e-11)
+    V = ws.V[:size].cpu().numpy().T
+    np.testing.assert_allclose(V, ov, atol=1e-10)
+    # orthonormality of the device basis
+    assert np.abs(V.conj().T @ V - np.eye(size)).max() < 1e-13
+
+
+def test_breakdown_on_invariant_subspace():
+    # H with a 3-dimensional invariant subspace containing the start vector -> basis of size 3
+    dims = (2, 2, 2)
+    rng = np.random.default_rng(2)
+    a = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+    h = engine.MatrixFreeHamiltonian(dims, [engine.LocalTerm((2,), a + a.conj().T)])
+    state = np.zeros(8, complex)
+    state[0] = 1.0
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=5, maximum_dimension=6, embedded_offset=2)
+    size = prop._basis(torch.from_numpy(state).cuda(), 5)
+    assert size == 2
+
+
+def test_candidate_errors_match_oracle():
+    h, o = _small_pair(4)
+    state = core.random_phase_initial_state(h.dimensions, 1, 1)
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=24, maximum_dimension=64)
+    oprop = oengine.AdaptiveLanczosPropagator(o, minimum_dimension=24, maximum_dimension=64)
+    size = prop._basis(torch.from_numpy(state).cuda(), 64)
+    _, oa, ob = oprop._basis(state, 64)
+    ladder = [24, 32, 40, 48, 56, 64]
+    for t in (0.01, 0.05, 0.2):
+        errs = prop._candidates(ladder, 4, t)
+        ws = prop._workspace()
+        for i, d in enumerate(ladder):
+            c, e = oprop._candidate(oa, ob, d, t, 1.0)
+            assert abs(errs[i] - e) <= 1e-12 + 1e-6 * e, (t, d, errs[i], e)
+            np.testing.assert_allclose(ws.coeffs[i, :d].cpu().numpy(), c, atol=1e-12)
+
+
+def _propagators(h, o, **kw):
+    return engine.AdaptiveLanczosPropagator(h, **kw), oengine.AdaptiveLanczosPropagator(o, **kw)
+
+
+def test_propagate_matches_oracle_schedule_and_values():
+    h, o = _small_pair(5)
+    state = core.random_phase_initial_state(h.dimensions, 11, 2)
+    times = np.linspace(0.0, 0.4, 321)
+    kw = dict(minimum_dimension=8, maximum_dimension=16, dimension_increment=4, embedded_offset=2,
+              state_error_budget=1e-7, initial_step_us=0.02, maximum_step_us=0.1)
+    p, op = _propagators(h, o, **kw)
+    a = p.propagate(state, times, engine.SingletPopulationObservable(h.dimensions))
+    b = op.propagate(state, times, oengine.SingletPopulationObservable(o.dimensions))
+    assert [(r[1], r[2], r[3]) for r in a.schedule] == [(r[1], r[2], r[3]) for r in b.schedule]
+    np.testing.assert_allclose([r[0] for r in a.schedule], [r[0] for r in b.schedule], rtol=1e-12)
+    np.testing.assert_allclose(a.observable_values, b.observable_values, atol=1e-10)
+    np.testing.assert_allclose(a.final_state, b.final_state, atol=1e-10)
+    np.testing.assert_allclose(a.state_error_estimates, b.state_error_estimates, rtol=1e-6, atol=1e-15)
+    assert a.stats.accepted_steps == b.stats.accepted_steps and a.stats.rejected_steps == b.stats.rejected_steps
+    assert a.stats.matvecs == b.stats.matvecs
+    # replaying the oracle's schedule gives the same result
+    c = p.propagate(state, times, engine.SingletPopulationObservable(h.dimensions), schedule=b.schedule)
+    np.testing.assert_allclose(c.observable_values, b.observable_values, atol=1e-10)
+
+
+def _ref_fixture():
+    sx, sy, sz = model.spin_operators(0.5)
+    dims = (2, 2, 2)
+    terms = (engine.LocalTerm((0,), 0.7 * sx + 0.2 * sz),
+             engine.LocalTerm((1, 2), 0.4 * np.kron(sy, sx) + 0.3 * np.kron(sz, sz)))
+    h = engine.MatrixFreeHamiltonian(dims, terms)
+    state = engine.singlet_initial_state(dims, np.random.default_rng(6))
+    return h, state, odense.dense_from_terms(dims, h.terms)
+
+
+def test_reference_lanczos_tests_on_gpu():
+    import scipy.linalg
+    # R/tests/test_statevector_baseline.py:207-234
+    h, state, d = _ref_fixture()
+    times = np.array([0.0, 0.03, 0.2, 0.7])
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=5, maximum_dimension=8, dimension_increment=1,
+                                            embedded_offset=2, state_error_budget=1e-11, initial_step_us=0.4,
+                                            maximum_step_us=0.4)
+    res = prop.propagate(state, times, lambda v: oengine.singlet_population(v, (2, 2, 2)), store_states=True)
+    exp = [oengine.singlet_population(scipy.linalg.expm(-1j * d * t) @ state, (2, 2, 2)) for t in times]
+    np.testing.assert_allclose(res.observable_values, exp, atol=2e-12)
+    np.testing.assert_allclose(res.final_state, scipy.linalg.expm(-1j * d * times[-1]) @ state, atol=2e-12)
+    np.testing.assert_allclose(res.states[-1], res.final_state, atol=2e-12)
+    assert res.stats.maximum_norm_error < 2e-12
+    # 236-252: step underflow
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=3, maximum_dimension=3, dimension_increment=1,
+                                            embedded_offset=1, state_error_budget=1e-30, initial_step_us=1.0,
+                                            maximum_step_us=1.0, minimum_step_us=0.2)
+    with pytest.raises(RuntimeError):
+        prop.propagate(state, [1.0], engine.SingletPopulationObservable((2, 2, 2)))
+    # 254-271: rejection recovery
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=3, maximum_dimension=3, dimension_increment=1,
+                                            embedded_offset=1, state_error_budget=1e-2, initial_step_us=1.0,
+                                            maximum_step_us=1.0, minimum_step_us=1e-6)
+    res = prop.propagate(state, [1.0], engine.SingletPopulationObservable((2, 2, 2)))
+    assert res.stats.rejected_steps > 0 and res.stats.accepted_steps > 0
+
+
+def test_maximum_basis_reused_after_growth():
+    # R/tests/test_statevector_baseline.py:273-295
+    rng = np.random.default_rng(4)
+    vals = rng.normal(size=(16, 16)) + 1j * rng.normal(size=(16, 16))
+    h = engine.MatrixFreeHamiltonian((2, 2, 2, 2), (engine.LocalTerm((0, 1, 2, 3), 20.0 * (vals + vals.conj().T) / 2),))
+    state = rng.normal(size=16) + 1j * rng.normal(size=16)
+    state /= np.linalg.norm(state)
+    res = engine.AdaptiveLanczosPropagator(h, minimum_dimension=4, maximum_dimension=12, dimension_increment=4,
+                                           embedded_offset=2, state_error_budget=1e-4, initial_step_us=0.1,
+                                           maximum_step_us=0.1, minimum_step_us=1e-8
+                                           ).propagate(state, [0.2], lambda v: float(abs(v[0]) ** 2))
+    attempts = res.stats.accepted_steps + res.stats.rejected_steps
+    assert res.stats.maximum_krylov_dimension == 12
+    assert res.stats.matvecs < attempts * (4 + 12)
+
+
+def test_small_system_yield_matches_oracle():
+    """576-dimensional random system, production Krylov rules, short horizon."""
+    sm = workloads.small_system(0)
+    h = sm.hamiltonian()
+    o = sm.hamiltonian(builder=ocore.build_hamiltonian)
+    cfg = core.SweepConfig(lifetime_us=1.0, survival_tail_bound=math.exp(-0.5))
+    ocfg = ocore.SweepConfig(lifetime_us=1.0, survival_tail_bound=math.exp(-0.5))
+    for sid in (0, 1):
+        s = core.random_phase_initial_state(h.dimensions, 0, sid)
+        a = core.propagate_trajectory(h, s, cfg)
+        b = ocore.run_trajectory(o, s, ocfg)
+        ra, rb = a["result"], b["result"]
+        assert [(x[1], x[2]) for x in ra.schedule] == [(x[1], x[2]) for x in rb.schedule]
+        np.testing.assert_allclose(ra.observable_values, rb.observable_values, atol=1e-9)
+        assert abs(a["yield"] - b["yield"]) < 1e-9
+        assert abs(a["quadrature_error"] - b["quadrature_error"]) < 1e-9
+        exact = odense.dense_singlet_yield_closed_form(o.explicit_matrix(), s, 1.0, cfg.time_grid_us[-1])
+        assert abs(a["yield"] - exact) < a["quadrature_error"] + 1e-6
+
+
+def test_full_system_short_horizon_matches_oracle():
+    """Production operator, dim 221,184, 0.02 us horizon (a few Lanczos steps)."""
+    h = core.build_hamiltonian(90.0, 0.05)
+    o = ocore.build_hamiltonian(90.0, 0.05)
+    times = np.arange(17) * 0.00125
+    s = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 0)
+    kw = dict(state_error_budget=2.4e-5 * times[-1] / 57.56, initial_step_us=0.05)
+    p, op = _propagators(h, o, **kw)
+    a = p.propagate(s, times, engine.SingletPopulationObservable(h.dimensions))
+    b = op.propagate(s, times, oengine.SingletPopulationObservable(o.dimensions))
+    assert [(x[1], x[2], x[3]) for x in a.schedule] == [(x[1], x[2], x[3]) for x in b.schedule]
+    np.testing.assert_allclose(a.observable_values, b.observable_values, atol=1e-10)
+    assert _rel(a.final_state, b.final_state) < 1e-10
