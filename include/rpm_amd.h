@@ -83,6 +83,16 @@ long long rpm_operator_dimension(const rpm_operator* op);
 int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, int batch,
                        long long ld, void* stream);
 
+/*
+ * H·psi kernel selection (no counterpart in the reference; for tests and
+ * benchmarks): 0 automatic (the tiled pair kernel when the shape allows it),
+ * 1 the per-amplitude gather kernel, 2 the tiled kernel (RPM_E_UNSUPPORTED
+ * when the operator shape has none).  rpm_operator_kernel reports the kernel
+ * the next apply will use: 1 gather, 2 tiled, 3 generic local-term kernel.
+ */
+int rpm_operator_set_kernel(rpm_operator* op, int kernel);
+int rpm_operator_kernel(const rpm_operator* op);
+
 /* Bytes of device workspace rpm_lanczos_basis needs for max dimension m. */
 size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m);
 

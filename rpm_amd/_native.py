@@ -38,6 +38,8 @@ SIGNATURES = {
     "rpm_operator_destroy": (ctypes.c_int, [_vp]),
     "rpm_operator_dimension": (_ll, [_vp]),
     "rpm_operator_apply": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _ll, _vp]),
+    "rpm_operator_set_kernel": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "rpm_operator_kernel": (ctypes.c_int, [_vp]),
     "rpm_lanczos_workspace_bytes": (_sz, [_vp, ctypes.c_int]),
     "rpm_lanczos_basis": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, _vp,
                                          _sz, _vp]),
@@ -47,7 +49,7 @@ SIGNATURES = {
     "rpm_singlet_gram": (ctypes.c_int, [_vp, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),
     "rpm_singlet_gram_workspace_bytes": (_sz, [ctypes.c_int, _ll]),
     "rpm_krylov_outputs": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp,
-                                          ctypes.c_double, _vp, _vp, _vp, _vp, _vp, _sz, _vp]),
+                                          ctypes.c_double, _vp, _vp, _vp, _vp, _sz, _vp]),
     "rpm_krylov_combine": (ctypes.c_int, [_vp, ctypes.c_int, _ll, _vp, _vp, _vp, _vp]),
     "rpm_norm": (ctypes.c_int, [_vp, _ll, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),
     "rpm_singlet_population": (ctypes.c_int, [_vp, _ll, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "rpm_common.hpp"
+#include "pair_tile.hpp"
 
 namespace rpm {
 
@@ -37,6 +38,9 @@ struct PairTables {
 
 struct PairOp {
   PairTables t;
+  TileTables tile;      // valid when use_tile
+  bool use_tile;
+  int kernel;           // 0 auto, 1 gather, 2 tile
   void* dev;            // single device allocation backing the tables
 };
 
@@ -183,6 +187,63 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     for (long long r = 0; r < NR; ++r) cB[e * NR + r] = cadd(fB[eB * NR + r], Pel(e, e ^ 1));
   }
 
+  // ---- tiled-kernel tables: the last NC = 2 left axes (spin-1/2) are closed in the tile
+  const int NC = 2;
+  bool tile_ok = nL >= NC;
+  for (int j = 0; j < NC && tile_ok; ++j) tile_ok = ldims[nL - 1 - j] == 2;
+  int KO = 0;
+  for (int i = 0; i < nL - NC && tile_ok; ++i) KO += ldims[i] - 1;
+  tile_ok = tile_ok && pair_tile_supported(static_cast<int>(NR), NC, KO, KR);
+  const int LG = 1 << NC;
+  const long long ngroups = tile_ok ? NL / LG : 0;
+  std::vector<unsigned char> flagR(KR > 0 ? KR : 1, 0), flagC(NC, 0);
+  std::vector<c128> coefC(NC * 2 * 4);
+  std::vector<int> ofsO(ngroups * KO > 0 ? ngroups * KO : 1, 0);
+  std::vector<c128> coefO(ngroups * KO > 0 ? ngroups * KO * 4 : 4);
+  std::vector<unsigned char> flagO(ngroups * KO > 0 ? ngroups * KO : 1, 0);
+  auto nz = [](c128 c) { return c.re != 0.0 || c.im != 0.0; };
+  auto flags4 = [&](const c128* C) {
+    int f = 0;
+    if (nz(C[0]) || nz(C[3])) f |= kSameE;
+    if (nz(C[1]) || nz(C[2])) f |= kFlipE;
+    return static_cast<unsigned char>(f);
+  };
+  if (tile_ok) {
+    for (long long r = 0; r < NR; ++r)
+      for (int k = 0; k < KR; ++k) flagR[k] |= flags4(&coefR[(r * KR + k) * 4]);
+    for (int j = 0; j < NC; ++j) {
+      const c128* B = blk[nL - 1 - j];      // d = 2, row = e*2 + m
+      for (int m = 0; m < 2; ++m) {
+        for (int e = 0; e < 2; ++e)
+          for (int e2 = 0; e2 < 2; ++e2) coefC[(j * 2 + m) * 4 + e * 2 + e2] = B[(e * 2 + m) * 4 + (e2 * 2 + (1 - m))];
+        flagC[j] |= flags4(&coefC[(j * 2 + m) * 4]);
+      }
+    }
+    std::vector<long long> lstride(nL);
+    {
+      long long s = 1;
+      for (int i = nL - 1; i >= 0; --i) { lstride[i] = s; s *= ldims[i]; }
+    }
+    for (long long g = 0; g < ngroups; ++g) {
+      const long long l0 = g * LG;
+      int k = 0;
+      for (int i = 0; i < nL - NC; ++i) {
+        const int d = ldims[i];
+        const int m = static_cast<int>((l0 / lstride[i]) % d);
+        const c128* B = blk[i];
+        for (int m2 = 0; m2 < d; ++m2) {
+          if (m2 == m) continue;
+          ofsO[g * KO + k] = static_cast<int>((m2 - m) * lstride[i]);
+          c128* C = &coefO[(g * KO + k) * 4];
+          for (int e = 0; e < 2; ++e)
+            for (int e2 = 0; e2 < 2; ++e2) C[e * 2 + e2] = B[(e * d + m) * (2 * d) + (e2 * d + m2)];
+          flagO[g * KO + k] = flags4(C);
+          ++k;
+        }
+      }
+    }
+  }
+
   PairOp* op = new PairOp();
   PairTables& T = op->t;
   T.NL = static_cast<int>(NL);
@@ -201,7 +262,15 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   const size_t o_cA = reserve(cA.size() * 16), o_cB = reserve(cB.size() * 16);
   const size_t o_offL = reserve(offL.size() * 4), o_offR = reserve(offR.size() * 4);
   const size_t o_coL = reserve(coefL.size() * 16), o_coR = reserve(coefR.size() * 16);
+  const size_t o_fR = reserve(flagR.size()), o_cC = reserve(coefC.size() * 16), o_fC = reserve(flagC.size());
+  const size_t o_oO = reserve(ofsO.size() * 4), o_cO = reserve(coefO.size() * 16), o_fO = reserve(flagO.size());
   std::vector<char> host(bytes);
+  std::memcpy(host.data() + o_fR, flagR.data(), flagR.size());
+  std::memcpy(host.data() + o_cC, coefC.data(), coefC.size() * 16);
+  std::memcpy(host.data() + o_fC, flagC.data(), flagC.size());
+  std::memcpy(host.data() + o_oO, ofsO.data(), ofsO.size() * 4);
+  std::memcpy(host.data() + o_cO, coefO.data(), coefO.size() * 16);
+  std::memcpy(host.data() + o_fO, flagO.data(), flagO.size());
   std::memcpy(host.data() + o_dA, dA.data(), dA.size() * 16);
   std::memcpy(host.data() + o_dB, dB.data(), dB.size() * 16);
   std::memcpy(host.data() + o_cA, cA.data(), cA.size() * 16);
@@ -229,6 +298,34 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   T.offR = reinterpret_cast<const int*>(base + o_offR);
   T.coefL = reinterpret_cast<const c128*>(base + o_coL);
   T.coefR = reinterpret_cast<const c128*>(base + o_coR);
+  op->use_tile = tile_ok;
+  op->kernel = 0;
+  if (tile_ok) {
+    TileTables& U = op->tile;
+    U.NL = T.NL;
+    U.NR = T.NR;
+    U.NC = NC;
+    U.KO = KO;
+    U.KR = KR;
+    U.ngroups = static_cast<int>(ngroups);
+    U.NLR = T.NLR;
+    U.dA = T.dA;
+    U.dB = T.dB;
+    U.cA = T.cA;
+    U.cB = T.cB;
+    U.offR = T.offR;
+    U.coefR = T.coefR;
+    U.flagR = reinterpret_cast<const unsigned char*>(base + o_fR);
+    U.coefC = reinterpret_cast<const c128*>(base + o_cC);
+    U.flagC = reinterpret_cast<const unsigned char*>(base + o_fC);
+    U.ofsO = reinterpret_cast<const int*>(base + o_oO);
+    U.coefO = reinterpret_cast<const c128*>(base + o_cO);
+    U.flagO = reinterpret_cast<const unsigned char*>(base + o_fO);
+    for (int e = 0; e < 4; ++e) {
+      U.pdiag[e] = T.pdiag[e];
+      U.pboth[e] = T.pboth[e];
+    }
+  }
   op->dev = dev;
   *out = op;
   return RPM_OK;
@@ -374,6 +471,11 @@ int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int bat
   if (!in || !out) return fail(RPM_E_INVALID, "NULL vector");
   if (op->kind == 1) {
     const PairOp* P = static_cast<const PairOp*>(op->impl);
+    if (P->use_tile && P->kernel != 1) {
+      if (static_cast<long long>(P->tile.ngroups) * batch > 0x7fffffffLL)
+        return fail(RPM_E_INVALID, "batch %d too large for one launch", batch);
+      return launch_pair_tile(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
+    }
     dim3 grid(ceil_div(P->t.NLR, 256), batch);
     hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, in_scale,
                        copy_out, active);
@@ -445,4 +547,20 @@ extern "C" int rpm_operator_apply(const rpm_operator* op, const void* d_in, void
   if (d_in == d_out) return fail(RPM_E_INVALID, "d_in and d_out must not overlap");
   return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld,
                            nullptr, nullptr, nullptr, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rpm_operator_set_kernel(rpm_operator* op, int kernel) {
+  if (!op || kernel < 0 || kernel > 2) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
+  if (op->kind != 1) return kernel == 2 ? fail(RPM_E_UNSUPPORTED, "tiled kernel needs a pair operator") : RPM_OK;
+  PairOp* P = static_cast<PairOp*>(op->impl);
+  if (kernel == 2 && !P->use_tile) return fail(RPM_E_UNSUPPORTED, "the operator shape has no tiled kernel");
+  P->kernel = kernel;
+  return RPM_OK;
+}
+
+extern "C" int rpm_operator_kernel(const rpm_operator* op) {
+  if (!op) return -1;
+  if (op->kind != 1) return 3;
+  const PairOp* P = static_cast<const PairOp*>(op->impl);
+  return (P->use_tile && P->kernel != 1) ? 2 : 1;
 }

@@ -1,0 +1,197 @@
+// Tiled pair stencil for the full-size two-radical layout (the production
+// FAD-TrpH operator and random-tensor systems of the same shape).
+//
+// Layout: psi[(e * NL + l) * NR + r], e = eA*2+eB.  One workgroup owns a tile
+// of all 4 electron states x LG = 2^NC consecutive left indices (the NC
+// lowest left axes, all spin-1/2, are "closed" inside the tile) x all NR right
+// indices; thread r holds the tile column psi[e][l][r] (4*LG amplitudes) in
+// registers and all 4*LG outputs in accumulators:
+//   * diagonal, electron flips and the closed left axes: registers only;
+//   * right-radical neighbours r': read from the tile staged in LDS;
+//   * the other left axes (open, KO neighbour offsets): read from global
+//     memory (L2), the same offsets and coefficients for the whole tile.
+// Every coefficient 2x2 electron block carries flags so that structurally
+// zero halves (same-electron or electron-flip) are skipped with uniform
+// branches.  Workgroups are mapped XCD-aware: all tiles of a vector run on one
+// XCD, so the re-read neighbours stay in that XCD's L2.
+#include <cstring>
+#include <vector>
+
+#include "rpm_common.hpp"
+#include "pair_tile.hpp"
+
+namespace rpm {
+
+__device__ inline void mac2x2_left(const c128* __restrict__ C, int flags, const c128 q[4], c128 acc[4]) {
+  // out[eA,eB] += sum_e' C[eA][e'] q[e',eB]   (q index e'*2+eB)
+  if (flags & kSameE) {
+    acc[0] = cfma(C[0], q[0], acc[0]);
+    acc[1] = cfma(C[0], q[1], acc[1]);
+    acc[2] = cfma(C[3], q[2], acc[2]);
+    acc[3] = cfma(C[3], q[3], acc[3]);
+  }
+  if (flags & kFlipE) {
+    acc[0] = cfma(C[1], q[2], acc[0]);
+    acc[1] = cfma(C[1], q[3], acc[1]);
+    acc[2] = cfma(C[2], q[0], acc[2]);
+    acc[3] = cfma(C[2], q[1], acc[3]);
+  }
+}
+
+__device__ inline void mac2x2_right(const c128 C[4], int flags, const c128 q[4], c128 acc[4]) {
+  // out[eA,eB] += sum_e' C[eB][e'] q[eA,e']   (q index eA*2+e')
+  if (flags & kSameE) {
+    acc[0] = cfma(C[0], q[0], acc[0]);
+    acc[1] = cfma(C[3], q[1], acc[1]);
+    acc[2] = cfma(C[0], q[2], acc[2]);
+    acc[3] = cfma(C[3], q[3], acc[3]);
+  }
+  if (flags & kFlipE) {
+    acc[0] = cfma(C[1], q[1], acc[0]);
+    acc[1] = cfma(C[2], q[0], acc[1]);
+    acc[2] = cfma(C[1], q[3], acc[2]);
+    acc[3] = cfma(C[2], q[2], acc[3]);
+  }
+}
+
+template <int NR, int NC, int KO, int KR>
+__global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __restrict__ in, c128* __restrict__ out,
+                                                  long long ld, int batch, const double* __restrict__ in_scale,
+                                                  c128* __restrict__ copy_out, const int* __restrict__ active) {
+  constexpr int LG = 1 << NC;
+  if (active && *active == 0) return;
+  __shared__ c128 tile[4 * LG * NR];
+  // XCD-aware mapping of the linear workgroup id onto (vector, group)
+  const int G = T.ngroups;
+  int b, grp;
+  {
+    const int g = blockIdx.x;
+    if (batch % kXcds == 0) {
+      const int x = g % kXcds, k = g / kXcds;
+      b = (k / G) * kXcds + x;
+      grp = k % G;
+    } else {
+      b = g / G;
+      grp = g % G;
+    }
+  }
+  const int r = threadIdx.x;
+  const long long S = T.NLR;
+  const c128* __restrict__ x = in + b * ld;
+  const int l0 = grp * LG;
+
+  c128 psi[LG][4];
+#pragma unroll
+  for (int l = 0; l < LG; ++l)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      psi[l][e] = x[e * S + static_cast<long long>(l0 + l) * NR + r];
+      tile[(e * LG + l) * NR + r] = psi[l][e];
+    }
+
+  // diagonal and electron flips
+  c128 acc[LG][4];
+  {
+    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
+    c128 cB[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) cB[e] = T.cB[e * NR + r];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
+        c128 a = cfma(d, psi[l][e], cmake(0.0, 0.0));
+        a = cfma(T.cA[e * T.NL + l0 + l], psi[l][e ^ 2], a);
+        a = cfma(cB[e], psi[l][e ^ 1], a);
+        a = cfma(T.pboth[e], psi[l][e ^ 3], a);
+        acc[l][e] = a;
+      }
+    }
+  }
+
+  // closed left axes: neighbour of l along closed axis j is l ^ (1 << j)
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      const int m = (l >> j) & 1;
+      const c128* C = T.coefC + (j * 2 + m) * 4;
+      mac2x2_left(C, T.flagC[j], psi[l ^ (1 << j)], acc[l]);
+    }
+  }
+
+  // open left axes: same offsets/coefficients for the whole tile, data from L2
+  {
+    const int* __restrict__ ofs = T.ofsO + grp * KO;
+    const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
+    const unsigned char* __restrict__ fl = T.flagO + grp * KO;
+#pragma unroll 1
+    for (int k = 0; k < KO; ++k) {
+      const long long o = static_cast<long long>(l0 + ofs[k]) * NR + r;
+      const int f = fl[k];
+      const c128* C = cof + k * 4;
+#pragma unroll
+      for (int l = 0; l < LG; ++l) {
+        c128 q[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = x[e * S + o + static_cast<long long>(l) * NR];
+        mac2x2_left(C, f, q, acc[l]);
+      }
+    }
+  }
+
+  __syncthreads();
+  // right-radical neighbours from the LDS tile
+#pragma unroll 1
+  for (int k = 0; k < KR; ++k) {
+    const int r2 = r + T.offR[r * KR + k];
+    c128 C[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C[i] = T.coefR[(r * KR + k) * 4 + i];
+    const int f = T.flagR[k];
+#pragma unroll
+    for (int l = 0; l < LG; ++l) {
+      c128 q[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l) * NR + r2];
+      mac2x2_right(C, f, q, acc[l]);
+    }
+  }
+
+  const double s = in_scale ? *in_scale : 1.0;
+  c128* __restrict__ y = out + b * ld;
+#pragma unroll
+  for (int l = 0; l < LG; ++l)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const long long idx = e * S + static_cast<long long>(l0 + l) * NR + r;
+      __builtin_nontemporal_store(acc[l][e].re * s, &y[idx].re);
+      __builtin_nontemporal_store(acc[l][e].im * s, &y[idx].im);
+    }
+  if (copy_out) {
+    c128* z = copy_out + b * ld;
+#pragma unroll
+    for (int l = 0; l < LG; ++l)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) z[e * S + static_cast<long long>(l0 + l) * NR + r] = cscale(psi[l][e], s);
+  }
+}
+
+int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                     c128* copy_out, const int* active, hipStream_t stream) {
+  const int blocks = T.ngroups * batch;
+  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
+    hipLaunchKernelGGL((k_pair_tile<192, 2, 7, 8>), dim3(blocks), dim3(192), 0, stream, T, in, out, ld, batch,
+                       in_scale, copy_out, active);
+  } else {
+    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  }
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+bool pair_tile_supported(int NR, int NC, int KO, int KR) { return NR == 192 && NC == 2 && KO == 7 && KR == 8; }
+
+}  // namespace rpm

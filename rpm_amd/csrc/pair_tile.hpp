@@ -1,0 +1,35 @@
+// Tables of the tiled pair stencil (see pair_tile.hip).
+#pragma once
+
+#include "rpm_common.hpp"
+
+namespace rpm {
+
+constexpr int kXcds = 8;         // MI355X: 8 accelerator complex dies, workgroups dispatched round-robin
+constexpr int kSameE = 1;        // coefficient block has a nonzero same-electron half
+constexpr int kFlipE = 2;        // ... a nonzero electron-flip half
+
+struct TileTables {
+  int NL, NR, NC, KO, KR, ngroups;
+  long long NLR;
+  const c128* dA;                  // [2][NL]
+  const c128* dB;                  // [2][NR]
+  const c128* cA;                  // [4][NL]
+  const c128* cB;                  // [4][NR]
+  const int* offR;                 // [NR][KR]
+  const c128* coefR;               // [NR][KR][4]
+  const unsigned char* flagR;      // [KR]
+  const c128* coefC;               // [NC][2][4]   closed left axes (spin-1/2), j = 0 is the last left axis
+  const unsigned char* flagC;      // [NC]
+  const int* ofsO;                 // [ngroups][KO]  left-index offset of each open-axis neighbour
+  const c128* coefO;               // [ngroups][KO][4]
+  const unsigned char* flagO;      // [ngroups][KO]
+  c128 pdiag[4];
+  c128 pboth[4];
+};
+
+int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                     c128* copy_out, const int* active, hipStream_t stream);
+bool pair_tile_supported(int NR, int NC, int KO, int KR);
+
+}  // namespace rpm

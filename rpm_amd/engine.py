@@ -86,6 +86,14 @@ class _DeviceOperator:
             pass
         self.handle = 0
 
+    def set_kernel(self, kernel: int) -> None:
+        """0 automatic, 1 per-amplitude gather kernel, 2 tiled pair kernel."""
+        nat.check(nat.load().rpm_operator_set_kernel(ctypes.c_void_p(self.handle), int(kernel)))
+
+    @property
+    def kernel(self) -> int:
+        return int(nat.load().rpm_operator_kernel(ctypes.c_void_p(self.handle)))
+
     def apply(self, x, out, batch=1, ld=None):
         nat.check(nat.load().rpm_operator_apply(
             ctypes.c_void_p(self.handle), ctypes.c_void_p(nat.ptr(x)), ctypes.c_void_p(nat.ptr(out)),

@@ -300,3 +300,54 @@ def test_full_system_short_horizon_matches_oracle():
     assert [(x[1], x[2], x[3]) for x in a.schedule] == [(x[1], x[2], x[3]) for x in b.schedule]
     np.testing.assert_allclose(a.observable_values, b.observable_values, atol=1e-10)
     assert _rel(a.final_state, b.final_state) < 1e-10
+
+
+@pytest.mark.parametrize("batch", [1, 3, 8, 16])
+def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch):
+    h, o = production_pair
+    dev = h._device_op()
+    assert dev.kernel == 2, "production operator should use the tiled pair kernel"
+    n = h.dimension
+    v = workloads.normal_vectors(n, batch, seed=10 + batch)
+    x = torch.from_numpy(v).cuda()
+    y_tile = torch.empty_like(x)
+    y_gather = torch.empty_like(x)
+    dev.apply(x, y_tile, batch=batch, ld=n)
+    dev.set_kernel(1)
+    try:
+        assert dev.kernel == 1
+        dev.apply(x, y_gather, batch=batch, ld=n)
+    finally:
+        dev.set_kernel(0)
+    yt, yg = y_tile.cpu().numpy(), y_gather.cpu().numpy()
+    for b in range(batch):
+        want = o.matvec(v[b]) if b < 2 else yg[b]
+        assert _rel(yt[b], want) <= HPSI_TOL, b
+        assert _rel(yg[b], want) <= HPSI_TOL, b
+
+
+def test_tiled_kernel_random_tensor_system():
+    sysr = workloads.full_random_system(seed=7)
+    h = sysr.hamiltonian(theta_deg=63.0, phi_deg=20.0)
+    o = sysr.hamiltonian(theta_deg=63.0, phi_deg=20.0, builder=ocore.build_hamiltonian)
+    assert h._device_op().kernel == 2
+    v = workloads.normal_vectors(h.dimension, 1, seed=8)[0]
+    assert _rel(h.matvec(v), o.matvec(v)) <= HPSI_TOL
+
+
+def test_tiled_kernel_normalising_copy():
+    # the fused normalisation used by the Lanczos basis: out = s*H in, copy = s*in
+    h = core.build_hamiltonian(10.0, 0.05)
+    o = ocore.build_hamiltonian(10.0, 0.05)
+    import ctypes
+    from rpm_amd import _native as nat
+    v = workloads.normal_vectors(h.dimension, 1, seed=9)[0] * 3.7
+    x = torch.from_numpy(v).cuda()
+    y = torch.empty_like(x)
+    c = torch.empty_like(x)
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=5, maximum_dimension=5, embedded_offset=2)
+    size = prop._basis(x, 2)
+    ws = prop._workspace()
+    v0 = ws.V[0].cpu().numpy()
+    np.testing.assert_allclose(v0, v / np.linalg.norm(v), atol=1e-15)
+    assert abs(ws.alpha[0].item() - np.vdot(v0, o.matvec(v0)).real) < 1e-12

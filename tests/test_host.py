@@ -42,6 +42,23 @@ def test_library_exports_every_declared_symbol():
     assert lib.rpm_version().decode().startswith("rpm_amd")
 
 
+def _declared_arg_counts():
+    text = open(os.path.join(ROOT, "include", "rpm_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(rpm_[a-z0-9_]+)\s*\(([^)]*)\)\s*;", text):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else args.count(",") + 1
+    return out
+
+
+def test_ctypes_signatures_match_header_argument_counts():
+    counts = _declared_arg_counts()
+    assert set(counts) == set(_native.SIGNATURES)
+    for name, n in counts.items():
+        assert len(_native.SIGNATURES[name][1]) == n, name
+
+
 def test_last_error_and_invalid_arguments_without_gpu():
     lib = _native.load()
     h = ctypes.c_void_p()
