@@ -124,7 +124,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": "batched H·ψ, production FAD-TrpH operator (theta=90 deg, 50 uT, "
                                "EH D/J=-3, Q on), dim 221184, complex128",
-                   "batch_vectors": B, "dimension": n, "kernel": "k_pair_apply"},
+                   "batch_vectors": B, "dimension": n,
+                   "kernel": {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_local_apply"}[h._device_op().kernel]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B)},
     }
