@@ -62,6 +62,31 @@ def measured_traffic(batch: int):
     return None if per_vector is None else per_vector * batch
 
 
+def propagate_probe(h, args) -> int:
+    """One production trajectory over a short horizon with the horizon-scaled error budget
+    of the reference's speed probe (R/rpm_full14_quadrupole_experiment.py:138-148)."""
+    import numpy as np
+    import torch
+    from rpm_amd import core
+    cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+    times = np.arange(int(round(args.horizon / 0.00125)) + 1) * 0.00125
+    prop = core.make_propagator(h, cfg, probe_horizon=float(times[-1]))
+    s = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 0)
+    obs = core.SingletPopulationObservable(h.dimensions)
+    prop.propagate(s, times[:3], obs)          # warm-up (allocations, first launches)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = prop.propagate(s, times, obs)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = res.stats
+    print(json.dumps({"mode": "propagate", "horizon_us": float(times[-1]), "seconds": dt,
+                      "simulated_us_per_s": float(times[-1]) / dt, "accepted": st.accepted_steps,
+                      "rejected": st.rejected_steps, "matvecs": st.matvecs, "max_krylov": st.maximum_krylov_dimension,
+                      "ms_per_matvec_incl_cgs2": dt * 1e3 / st.matvecs}), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,6 +94,10 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
+    ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
+    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate"),
+                    help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -86,11 +115,26 @@ def main() -> int:
         return 2
     h = core.build_hamiltonian(90.0, 0.05)
     assert h.uses_pair_stencil
+    if args.mode == "propagate":
+        return propagate_probe(h, args)
+    h._device_op().set_kernel(args.kernel)
     n = h.dimension
     B = args.batch
     print(f"[bench] building {B} synthetic vectors of dim {n}", file=sys.stderr, flush=True)
     x = torch.from_numpy(workloads.normal_vectors(n, B, seed=0)).cuda()
     y = torch.empty_like(x)
+    if args.mode == "copy":
+        for _ in range(args.warmup):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"mode": "copy", "bytes_per_step": 2 * 16 * n * B, "ms_per_step": dt * 1e3 / args.steps,
+                          "GBps": 2 * 16 * n * B * args.steps / dt / 1e9}), flush=True)
+        return 0
     for _ in range(args.warmup):
         h.apply_device(x, y, batch=B, ld=n)
     torch.cuda.synchronize()
@@ -125,7 +169,7 @@ def main() -> int:
         "config": {"workload": "batched H·ψ, production FAD-TrpH operator (theta=90 deg, 50 uT, "
                                "EH D/J=-3, Q on), dim 221184, complex128",
                    "batch_vectors": B, "dimension": n,
-                   "kernel": {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_local_apply"}[h._device_op().kernel]},
+                   "kernel": {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 4: "k_pair_tile4", 9: "k_local_apply"}[h._device_op().kernel]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B)},
     }

@@ -85,10 +85,12 @@ int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, in
 
 /*
  * H·psi kernel selection (no counterpart in the reference; for tests and
- * benchmarks): 0 automatic (the tiled pair kernel when the shape allows it),
- * 1 the per-amplitude gather kernel, 2 the tiled kernel (RPM_E_UNSUPPORTED
- * when the operator shape has none).  rpm_operator_kernel reports the kernel
- * the next apply will use: 1 gather, 2 tiled, 3 generic local-term kernel.
+ * benchmarks): 0 automatic (a tiled pair kernel when the shape allows it),
+ * 1 the per-amplitude gather kernel, 2 the register-tiled kernel, 3 the
+ * LDS-tiled kernel, 4 the LDS-tiled kernel with all global loads issued up
+ * front (2-4 give RPM_E_UNSUPPORTED when the operator shape has no tiled
+ * form).  rpm_operator_kernel reports the kernel the next apply will use:
+ * 1-4 as above, 9 the generic local-term kernel.
  */
 int rpm_operator_set_kernel(rpm_operator* op, int kernel);
 int rpm_operator_kernel(const rpm_operator* op);

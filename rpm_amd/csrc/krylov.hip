@@ -1,13 +1,25 @@
-// Small-matrix side of the Krylov propagator, on the device:
-//   * symmetric tridiagonal eigendecomposition T = Z diag(w) Z^T by the
-//     implicit QL algorithm with Wilkinson-type shifts (textbook method, e.g.
-//     Wilkinson & Reinsch, Handbook for Automatic Computation II, tql2), one
-//     wavefront per matrix: lane i owns d[i], e[i] and row i of Z;
-//   * c(t) = Z (exp(-i w t) * Z[0,:]) -- the projected exponential of
-//     R/rpm_statevector_engine.py:569-609;
-//   * embedded error estimates of the candidate ladder
-//     (R/rpm_statevector_engine.py:690-715) and the per-output-time singlet
-//     values and point errors of an accepted step (861-902).
+// Small-matrix side of the Krylov propagator, on the device.
+//
+// The reference evaluates the projected exponential c(t) = exp(-i T t) e_1 of
+// the Lanczos tridiagonal T by a full eigendecomposition
+// (R/rpm_statevector_engine.py:569-609).  Here c(t) is evaluated directly by
+// the Chebyshev expansion of the exponential (Tal-Ezer & Kosloff, J. Chem.
+// Phys. 81, 3967 (1984)):
+//   T = c0 + h X with spec(X) in [-1, 1] (Gershgorin bounds),
+//   exp(-i T t) v = exp(-i c0 t) sum_k a_k T_k(X) v,  a_0 = J_0(ht),
+//   a_k = 2 (-i)^k J_k(ht),
+// with the Bessel values from Miller's backward recurrence normalised by
+// J_0 + 2 sum J_2k = 1.  T is real, so T_k(X) e_1 is a real vector and the
+// recurrence v_{k+1} = 2 X v_k - v_{k-1} costs a few lane operations per term:
+// lane j owns component j and the tridiagonal neighbours come from DPP wave
+// shifts.  The series stops where the Bessel weights are below 1e-18
+// (k > 1.5 ht + 40); for ht beyond kSplitZ the time is split into equal
+// sub-steps applied in sequence.  One wavefront evaluates one
+// (dimension, time).
+//
+// Kernels: embedded error estimates of a candidate ladder
+// (R/rpm_statevector_engine.py:690-715) and the per-output-time singlet values
+// and point errors of an accepted step (861-902).
 #include <cmath>
 
 #include "rpm_common.hpp"
@@ -16,207 +28,181 @@ namespace rpm {
 
 constexpr int kMaxKrylov = 64;
 constexpr int kMaxCand = 16;
-
-struct Slot {          // one decomposition in scratch
-  double w[kMaxKrylov];                    // eigenvalues
-  double zt[kMaxKrylov * kMaxKrylov];      // Zt[k*64 + i] = Z[i][k]
-  c128 c[kMaxKrylov];                      // c(t) at the requested time
-};
+constexpr double kSplitZ = 1300.0;                               // largest ht per sub-step
+constexpr int kMaxTerms = static_cast<int>(1.5 * kSplitZ) + 41;  // Bessel terms per sub-step
 
 struct CandArgs {
   int ncand;
   int dims[kMaxCand];
 };
 
-__device__ inline double lane_get(double v, int i) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), i);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), i);
+// lane i <- lane i-1 (0 for lane 0)
+__device__ inline double shift_up(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xf, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+// lane i <- lane i+1 (0 for lane 63)
+__device__ inline double shift_down(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x130, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x130, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 
-// Eigendecomposition of the leading n x n block (n <= 64) of the tridiagonal
-// (alpha, beta).  Must be called by one full wavefront.  zrow: LDS row storage
-// for this wave, zrow[i * 65 + k] = Z[i][k].  Returns w in lane i's `wout`.
-__device__ bool ql_eig(int n, const double* __restrict__ alpha, const double* __restrict__ beta,
-                       double* zrow, double& wout) {
+__device__ inline double wave_min(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmin(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+  return v;
+}
+
+// Bessel J_0..J_K(z), z > 0, into bess[0..K] by Miller's backward recurrence (one lane).
+__device__ void bessel_series(double z, int K, double* bess) {
+  const int N = K + 60 + static_cast<int>(sqrt(40.0 * (K + 1)));
+  const double two_over_z = 2.0 / z;
+  double jp1 = 0.0, j = 1e-300, norm = 0.0;
+  for (int k = N; k >= 1; --k) {
+    const double jm1 = k * two_over_z * j - jp1;
+    jp1 = j;
+    j = jm1;                                  // j = J_{k-1} (unnormalised)
+    if (k - 1 <= K) bess[k - 1] = j;
+    if (((k - 1) & 1) == 0 && k - 1 > 0) norm += 2.0 * j;
+    if (fabs(j) > 1e250) {                    // rescale everything kept so far
+      j *= 1e-250;
+      jp1 *= 1e-250;
+      norm *= 1e-250;
+      for (int i = k - 1; i <= K; ++i) bess[i] *= 1e-250;
+    }
+  }
+  norm += j;                                  // + J_0
+  const double inv = 1.0 / norm;
+  for (int i = 0; i <= K; ++i) bess[i] *= inv;
+}
+
+// One real Chebyshev recurrence: returns sum_k w_k T_k(X) v0 split into the
+// (-i)^k classes: re = sum_{k even} +-w_k T_k v0, im = sum_{k odd} -+w_k T_k v0.
+__device__ inline void cheb_real(double v0, double dx, double ox, double oxp, const double* bess, int K,
+                                 double& re, double& im) {
+  double vprev = v0;
+  double vcur = dx * vprev + oxp * shift_up(vprev) + ox * shift_down(vprev);
+  re = bess[0] * vprev;
+  im = -2.0 * bess[1] * vcur;
+  for (int k = 2; k <= K; ++k) {
+    const double xv = dx * vcur + oxp * shift_up(vcur) + ox * shift_down(vcur);
+    const double vnext = 2.0 * xv - vprev;
+    vprev = vcur;
+    vcur = vnext;
+    const double w = 2.0 * bess[k];
+    switch (k & 3) {                                          // (-i)^k
+      case 0: re += w * vcur; break;
+      case 1: im -= w * vcur; break;
+      case 2: re -= w * vcur; break;
+      default: im += w * vcur; break;
+    }
+  }
+}
+
+// c = exp(-i T_n t) e_1 for the leading n x n block of (alpha, beta), t >= 0;
+// lane j returns component j (0 for j >= n).  One full wavefront; bess: LDS
+// scratch of kMaxTerms + 1 doubles private to the wave.
+__device__ c128 cheb_expm_e1(int n, const double* __restrict__ alpha, const double* __restrict__ beta, double t,
+                             double* bess) {
   const int lane = threadIdx.x % kWave;
-  double d = lane < n ? alpha[lane] : 0.0;
-  double e = lane < n - 1 ? beta[lane] : 0.0;
-  for (int k = 0; k < n; ++k) zrow[lane * 65 + k] = (lane == k) ? 1.0 : 0.0;
-  bool ok = true;
-  const double eps = 2.220446049250313e-16;
-  for (int l = 0; l < n; ++l) {
-    int iter = 0;
-    for (;;) {
-      int m = l;
-      for (; m < n - 1; ++m) {
-        const double dd = fabs(lane_get(d, m)) + fabs(lane_get(d, m + 1));
-        if (fabs(lane_get(e, m)) <= eps * dd) break;
-      }
-      if (m == l) break;
-      if (++iter > 80) {
-        ok = false;
-        break;
-      }
-      const double dl = lane_get(d, l), el = lane_get(e, l);
-      double g = (lane_get(d, l + 1) - dl) / (2.0 * el);
-      double r = sqrt(g * g + 1.0);
-      g = lane_get(d, m) - dl + el / (g + (g >= 0.0 ? r : -r));
-      double s = 1.0, c = 1.0, p = 0.0;
-      bool deflated = false;
-      for (int i = m - 1; i >= l; --i) {
-        const double ei = lane_get(e, i);
-        const double f = s * ei, b = c * ei;
-        r = sqrt(f * f + g * g);
-        if (lane == i + 1) e = r;
-        if (r == 0.0) {
-          const double dn = lane_get(d, i + 1) - p;
-          if (lane == i + 1) d = dn;
-          if (lane == m) e = 0.0;
-          deflated = true;
-          break;
-        }
-        const double inv = 1.0 / r;
-        s = f * inv;
-        c = g * inv;
-        g = lane_get(d, i + 1) - p;
-        r = (lane_get(d, i) - g) * s + 2.0 * c * b;
-        p = s * r;
-        if (lane == i + 1) d = g + p;
-        g = c * r - b;
-        if (lane < n) {
-          const double zi = zrow[lane * 65 + i], zj = zrow[lane * 65 + i + 1];
-          zrow[lane * 65 + i + 1] = s * zi + c * zj;
-          zrow[lane * 65 + i] = c * zi - s * zj;
-        }
-      }
-      if (deflated) continue;
-      if (lane == l) {
-        d -= p;
-        e = g;
-      }
-      if (lane == m) e = 0.0;
-    }
-    if (!ok) break;
-  }
-  wout = d;
-  return ok;
-}
-
-// One block (one wave) per (candidate, high/low).  Writes the decomposition
-// and c(t) into slot[cand*2 + which].
-__global__ void __launch_bounds__(64) k_tridiag_expm(const double* __restrict__ alpha, const double* __restrict__ beta,
-                                                     const int* __restrict__ info, CandArgs args, int offset, double t,
-                                                     Slot* __restrict__ slots, int* __restrict__ status) {
-  __shared__ double zrow[kMaxKrylov * 65];
-  __shared__ c128 u[kMaxKrylov];
-  const int cand = blockIdx.x, which = blockIdx.y;
-  const int lane = threadIdx.x;
-  const int size = info ? info[0] : args.dims[cand];
-  const int dim = min(args.dims[cand], size);
-  const int low = max(1, dim - offset);
-  if (which == 1 && low == dim) return;
-  const int n = which == 0 ? dim : low;
-  if (n < 1) return;
-  double w = 0.0;
-  const bool ok = ql_eig(n, alpha, beta, zrow, w);
-  if (!ok && lane == 0) atomicOr(status, 1);
-  __syncthreads();   // rows of Z written by other lanes are read below
-  Slot* S = slots + cand * 2 + which;
-  if (lane < n) {
-    S->w[lane] = w;
-    for (int k = 0; k < n; ++k) S->zt[k * kMaxKrylov + lane] = zrow[lane * 65 + k];
+  const double a = lane < n ? alpha[lane] : 0.0;
+  const double b = lane < n - 1 ? beta[lane] : 0.0;          // coupling (lane, lane+1)
+  const double bp = shift_up(b);                              // coupling (lane-1, lane)
+  const double lo = wave_min(lane < n ? a - fabs(bp) - fabs(b) : 1e300);
+  const double hi = wave_max(lane < n ? a + fabs(bp) + fabs(b) : -1e300);
+  const double c0 = 0.5 * (lo + hi);
+  double h = 0.5 * (hi - lo);
+  if (!(h > 0.0) || t == 0.0) {
     double sn, cs;
-    sincos(w * t, &sn, &cs);
-    const double z0 = zrow[0 * 65 + lane];   // Z[0][lane]
-    u[lane] = cmake(z0 * cs, -z0 * sn);
+    sincos(c0 * t, &sn, &cs);
+    return lane == 0 ? cmake(cs, -sn) : cmake(0.0, 0.0);
   }
+  h *= 1.0 + 1e-12;
+  const double ih = 1.0 / h;
+  const double dx = lane < n ? (a - c0) * ih : 0.0;
+  const double ox = b * ih, oxp = bp * ih;
+  const double ztot = h * t;
+  const int nsub = ztot > kSplitZ ? static_cast<int>(ceil(ztot / kSplitZ)) : 1;
+  const double tsub = t / nsub, z = h * tsub;
+  const int K = static_cast<int>(1.5 * z) + 40;
+  if (lane == 0) bessel_series(z, K, bess);
   __syncthreads();
-  if (lane < n) {
-    c128 acc{0.0, 0.0};
-    for (int k = 0; k < n; ++k) {
-      const double z = zrow[lane * 65 + k];
-      acc.re = fma(z, u[k].re, acc.re);
-      acc.im = fma(z, u[k].im, acc.im);
-    }
-    S->c[lane] = acc;
+  double re, im;
+  cheb_real(lane == 0 ? 1.0 : 0.0, dx, ox, oxp, bess, K, re, im);
+  for (int s = 1; s < nsub; ++s) {
+    // apply the same series to the complex vector (re + i im): X is real
+    double rr, ri, ir, ii;
+    cheb_real(re, dx, ox, oxp, bess, K, rr, ri);
+    cheb_real(im, dx, ox, oxp, bess, K, ir, ii);
+    re = rr - ii;
+    im = ri + ir;
   }
+  if (lane >= n) return cmake(0.0, 0.0);
+  double sn, cs;
+  sincos(c0 * t, &sn, &cs);                                   // exp(-i c0 t) = cs - i sn
+  return cmake(cs * re + sn * im, cs * im - sn * re);
 }
 
-__global__ void __launch_bounds__(64) k_cand_err(const int* __restrict__ info, const double* __restrict__ norm,
-                                                 CandArgs args, int offset, double safety, const Slot* __restrict__ slots,
-                                                 int m_stride, double* __restrict__ err, c128* __restrict__ coeffs) {
-  const int cand = blockIdx.x, lane = threadIdx.x;
+// One block (two wavefronts: the candidate dimension and its embedded low dimension) per candidate.
+__global__ void __launch_bounds__(128) k_candidates(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                    const int* __restrict__ info, const double* __restrict__ norm,
+                                                    CandArgs args, int offset, double safety, double t, int m_stride,
+                                                    double* __restrict__ err, c128* __restrict__ coeffs) {
+  __shared__ double bess[2][kMaxTerms + 1];
+  __shared__ c128 cl[kMaxKrylov];
+  const int cand = blockIdx.x;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int size = info[0];
   const int dim = min(args.dims[cand], size);
   const int low = max(1, dim - offset);
-  const Slot* H = slots + cand * 2;
-  const Slot* L = slots + cand * 2 + 1;
+  const c128 c = cheb_expm_e1(wave == 0 ? dim : low, alpha, beta, t, bess[wave]);
+  if (wave == 1) cl[lane] = c;
+  __syncthreads();
+  if (wave != 0) return;
+  for (int i = lane; i < m_stride; i += kWave)
+    coeffs[static_cast<long long>(cand) * m_stride + i] = (i < dim) ? c : cmake(0.0, 0.0);
   double s2 = 0.0;
-  for (int i = lane; i < m_stride; i += kWave) {
-    const c128 ch = i < dim ? H->c[i] : cmake(0.0, 0.0);
-    coeffs[static_cast<long long>(cand) * m_stride + i] = ch;
-    if (low != dim && i < dim) {
-      c128 df = ch;
-      if (i < low) df = csub(df, L->c[i]);
-      s2 = fma(df.re, df.re, s2);
-      s2 = fma(df.im, df.im, s2);
-    }
+  if (low != dim && lane < dim) {
+    c128 df = c;
+    if (lane < low) df = csub(df, cl[lane]);
+    s2 = df.re * df.re + df.im * df.im;
   }
   s2 = wave_sum(s2);
   if (lane == 0) err[cand] = (low == dim) ? 0.0 : safety * norm[0] * sqrt(s2);
 }
 
-// One block (one wave) per output time.
-__global__ void __launch_bounds__(64) k_outputs(const double* __restrict__ norm, const c128* __restrict__ P, int sel,
-                                                int low, const double* __restrict__ offsets, double safety,
-                                                const Slot* __restrict__ hs, const Slot* __restrict__ ls,
-                                                double* __restrict__ values, double* __restrict__ point_err,
-                                                c128* __restrict__ coeffs_out) {
-  __shared__ c128 uh[kMaxKrylov], ul[kMaxKrylov], ch[kMaxKrylov];
-  const int lane = threadIdx.x;
+// One block (two wavefronts: sel and low) per output time.
+__global__ void __launch_bounds__(128) k_outputs(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                 const double* __restrict__ norm, const c128* __restrict__ P, int sel,
+                                                 int low, const double* __restrict__ offsets, double safety,
+                                                 double* __restrict__ values, double* __restrict__ point_err,
+                                                 c128* __restrict__ coeffs_out) {
+  __shared__ double bess[2][kMaxTerms + 1];
+  __shared__ c128 ch[kMaxKrylov], cl[kMaxKrylov];
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const double t = offsets[blockIdx.x];
-  if (lane < sel) {
-    double sn, cs;
-    sincos(hs->w[lane] * t, &sn, &cs);
-    const double z0 = hs->zt[lane * kMaxKrylov + 0];
-    uh[lane] = cmake(z0 * cs, -z0 * sn);
-  }
-  if (low > 0 && lane < low) {
-    double sn, cs;
-    sincos(ls->w[lane] * t, &sn, &cs);
-    const double z0 = ls->zt[lane * kMaxKrylov + 0];
-    ul[lane] = cmake(z0 * cs, -z0 * sn);
-  }
+  const c128 c = cheb_expm_e1(wave == 0 ? sel : (low > 0 ? low : 1), alpha, beta, t, bess[wave]);
+  if (wave == 0) ch[lane] = c;
+  else cl[lane] = c;
   __syncthreads();
-  c128 c{0.0, 0.0}, cl{0.0, 0.0};
-  if (lane < sel) {
-    for (int k = 0; k < sel; ++k) {
-      const double z = hs->zt[k * kMaxKrylov + lane];
-      c.re = fma(z, uh[k].re, c.re);
-      c.im = fma(z, uh[k].im, c.im);
-    }
-    ch[lane] = c;
-    if (coeffs_out) coeffs_out[static_cast<long long>(blockIdx.x) * sel + lane] = c;
-  }
-  if (low > 0 && lane < low) {
-    for (int k = 0; k < low; ++k) {
-      const double z = ls->zt[k * kMaxKrylov + lane];
-      cl.re = fma(z, ul[k].re, cl.re);
-      cl.im = fma(z, ul[k].im, cl.im);
-    }
-  }
-  __syncthreads();
-  // (P c)_i = sum_j P[i][j] c_j = sum_j conj(P[j][i]) c_j   (P Hermitian)
+  if (wave != 0) return;
   double q = 0.0, e2 = 0.0;
   if (lane < sel) {
+    if (coeffs_out) coeffs_out[static_cast<long long>(blockIdx.x) * sel + lane] = c;
+    // (P c)_i = sum_j P[i][j] c_j = sum_j conj(P[j][i]) c_j   (P Hermitian)
     c128 pc{0.0, 0.0};
     for (int j = 0; j < sel; ++j) pc = cfmaconj(P[j * sel + lane], ch[j], pc);
-    // Re(conj(c_i) (Pc)_i)
     q = c.re * pc.re + c.im * pc.im;
     if (low > 0) {
       c128 df = c;
-      if (lane < low) df = csub(df, cl);
+      if (lane < low) df = csub(df, cl[lane]);
       e2 = df.re * df.re + df.im * df.im;
     }
   }
@@ -234,34 +220,30 @@ __global__ void __launch_bounds__(64) k_outputs(const double* __restrict__ norm,
 using namespace rpm;
 
 extern "C" size_t rpm_krylov_scratch_bytes(int ncand, int m_stride) {
+  (void)ncand;
   (void)m_stride;
-  const int n = ncand > 0 ? ncand : 1;
-  return static_cast<size_t>(2 * n) * sizeof(Slot) + 256;
+  return 256;
 }
 
 extern "C" int rpm_krylov_candidates(const double* d_alpha, const double* d_beta, const int* d_info,
                                      const double* d_norm, int ncand, const int* dims, int offset, double safety,
                                      double t, int m_stride, double* d_err, void* d_coeffs, void* d_scratch,
                                      void* stream) {
-  if (!d_alpha || !d_beta || !d_info || !d_norm || !dims || !d_err || !d_coeffs || !d_scratch)
+  (void)d_scratch;
+  if (!d_alpha || !d_beta || !d_info || !d_norm || !dims || !d_err || !d_coeffs)
     return fail(RPM_E_INVALID, "rpm_krylov_candidates: NULL argument");
   if (ncand < 1 || ncand > kMaxCand) return fail(RPM_E_INVALID, "rpm_krylov_candidates: 1..%d candidates", kMaxCand);
   if (m_stride < 1 || m_stride > kMaxKrylov)
     return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported on the device", kMaxKrylov);
+  if (!(t >= 0.0)) return fail(RPM_E_INVALID, "rpm_krylov_candidates: negative time");
   CandArgs a{};
   a.ncand = ncand;
   for (int i = 0; i < ncand; ++i) {
     if (dims[i] < 1 || dims[i] > m_stride) return fail(RPM_E_INVALID, "candidate dimension %d out of range", dims[i]);
     a.dims[i] = dims[i];
   }
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  Slot* slots = static_cast<Slot*>(d_scratch);
-  int* status = reinterpret_cast<int*>(static_cast<char*>(d_scratch) + 2 * ncand * sizeof(Slot));
-  hipLaunchKernelGGL(k_tridiag_expm, dim3(ncand, 2), dim3(64), 0, s, d_alpha, d_beta, d_info, a, offset, t, slots,
-                     status);
-  RPM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_cand_err, dim3(ncand), dim3(64), 0, s, d_info, d_norm, a, offset, safety, slots, m_stride,
-                     d_err, static_cast<c128*>(d_coeffs));
+  hipLaunchKernelGGL(k_candidates, dim3(ncand), dim3(128), 0, static_cast<hipStream_t>(stream), d_alpha, d_beta, d_info,
+                     d_norm, a, offset, safety, t, m_stride, d_err, static_cast<c128*>(d_coeffs));
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }
@@ -270,25 +252,16 @@ extern "C" int rpm_krylov_outputs(const double* d_alpha, const double* d_beta, c
                                   int sel, int low, int nt, const double* d_offsets, double safety, double* d_values,
                                   double* d_point_err, void* d_coeffs_out, void* d_scratch, size_t scratch_bytes,
                                   void* stream) {
-  if (!d_alpha || !d_beta || !d_norm || !d_P || !d_offsets || !d_values || !d_point_err || !d_scratch)
+  (void)d_scratch;
+  (void)scratch_bytes;
+  if (!d_alpha || !d_beta || !d_norm || !d_P || !d_offsets || !d_values || !d_point_err)
     return fail(RPM_E_INVALID, "rpm_krylov_outputs: NULL argument");
   if (sel < 1 || sel > kMaxKrylov || low < 0 || (low > 0 && low >= sel) || nt < 0)
     return fail(RPM_E_INVALID, "rpm_krylov_outputs: sel=%d low=%d", sel, low);
-  if (scratch_bytes < rpm_krylov_scratch_bytes(1, sel)) return fail(RPM_E_INVALID, "rpm_krylov_outputs: scratch");
   if (nt == 0) return RPM_OK;
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  Slot* slots = static_cast<Slot*>(d_scratch);
-  int* status = reinterpret_cast<int*>(static_cast<char*>(d_scratch) + 2 * sizeof(Slot));
-  // decompose T_sel (slot 0) and T_low (slot 1): a one-candidate ladder with offset sel-low
-  CandArgs a{};
-  a.ncand = 1;
-  a.dims[0] = sel;
-  const int offset = low > 0 ? sel - low : 0;
-  hipLaunchKernelGGL(k_tridiag_expm, dim3(1, low > 0 ? 2 : 1), dim3(64), 0, s, d_alpha, d_beta, nullptr, a, offset,
-                     0.0, slots, status);
-  RPM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_outputs, dim3(nt), dim3(64), 0, s, d_norm, static_cast<const c128*>(d_P), sel, low, d_offsets,
-                     safety, slots, slots + 1, d_values, d_point_err, static_cast<c128*>(d_coeffs_out));
+  hipLaunchKernelGGL(k_outputs, dim3(nt), dim3(128), 0, static_cast<hipStream_t>(stream), d_alpha, d_beta, d_norm,
+                     static_cast<const c128*>(d_P), sel, low, d_offsets, safety, d_values, d_point_err,
+                     static_cast<c128*>(d_coeffs_out));
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

@@ -148,41 +148,98 @@ __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict
   }
 }
 
-// w -= sum_c V_c h_c over the chunk; optional partial sums of |w|^2.
+// w -= sum_c V_c h_c over the chunk (one element per thread); optional partial sums of |w|^2.
+constexpr int kUpdChunk = kThreads;
 __global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V, long long n, int k,
                                                      const c128* __restrict__ h, c128* __restrict__ w,
                                                      double* __restrict__ norm_partial, const BasisCtl* ctl) {
   if (ctl && ctl->active == 0) return;
   __shared__ double sm[kThreads / kWave];
-  const long long base = static_cast<long long>(blockIdx.x) * kChunk + threadIdx.x;
-  c128 wv[kEPT];
-#pragma unroll
-  for (int e = 0; e < kEPT; ++e) {
-    const long long i = base + static_cast<long long>(e) * kThreads;
-    wv[e] = i < n ? w[i] : cmake(0.0, 0.0);
-  }
-  for (int c = 0; c < k; ++c) {
-    const c128 hc = h[c];
-    const c128* v = V + static_cast<long long>(c) * n;
-#pragma unroll
-    for (int e = 0; e < kEPT; ++e) {
-      const long long i = base + static_cast<long long>(e) * kThreads;
-      if (i < n) wv[e] = cfms(v[i], hc, wv[e]);
+  const long long i = static_cast<long long>(blockIdx.x) * kUpdChunk + threadIdx.x;
+  c128 wv = i < n ? w[i] : cmake(0.0, 0.0);
+  if (i < n) {
+    int c = 0;
+    for (; c + 4 <= k; c += 4) {
+      const c128 v0 = V[static_cast<long long>(c) * n + i], v1 = V[static_cast<long long>(c + 1) * n + i];
+      const c128 v2 = V[static_cast<long long>(c + 2) * n + i], v3 = V[static_cast<long long>(c + 3) * n + i];
+      wv = cfms(v0, h[c], wv);
+      wv = cfms(v1, h[c + 1], wv);
+      wv = cfms(v2, h[c + 2], wv);
+      wv = cfms(v3, h[c + 3], wv);
     }
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int e = 0; e < kEPT; ++e) {
-    const long long i = base + static_cast<long long>(e) * kThreads;
-    if (i < n) {
-      w[i] = wv[e];
-      s = fma(wv[e].re, wv[e].re, s);
-      s = fma(wv[e].im, wv[e].im, s);
-    }
+    for (; c < k; ++c) wv = cfms(V[static_cast<long long>(c) * n + i], h[c], wv);
+    w[i] = wv;
   }
   if (norm_partial) {
+    double s = fma(wv.re, wv.re, wv.im * wv.im);
     s = block_sum(s, sm);
     if (threadIdx.x == 0) norm_partial[blockIdx.x] = s;
+  }
+}
+
+// Fused CGS pass over an LDS-staged chunk of the basis: optionally
+// w <- w - V h (update), then partial[c][blk] = sum_x conj(V_c[x]) w[x] for
+// all c < k.  Each block walks kGsPer chunks of kGsChunk elements; V is read
+// from HBM once per pass.  k <= kGsMaxK.
+constexpr int kGsChunk = 32;
+constexpr int kGsPer = 4;
+constexpr int kGsMaxK = 64;
+__global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, long long n, int k,
+                                                 const c128* __restrict__ h, c128* __restrict__ w,
+                                                 c128* __restrict__ partial, const BasisCtl* ctl) {
+  if (ctl && ctl->active == 0) return;
+  __shared__ c128 vs[kGsMaxK][kGsChunk + 1];
+  __shared__ c128 ws[kGsChunk];
+  __shared__ c128 red[256];
+  __shared__ c128 hs[kGsMaxK];
+  const int tid = threadIdx.x;
+  const int nblk = gridDim.x;
+  if (h && tid < k) hs[tid] = h[tid];
+  // dot mapping: column c = tid % 64, part q = tid / 64 (4 parts of 8 elements)
+  const int dc = tid % 64, dq = tid / 64;
+  c128 dot{0.0, 0.0};
+  for (int it = 0; it < kGsPer; ++it) {
+    const long long x0 = (static_cast<long long>(blockIdx.x) * kGsPer + it) * kGsChunk;
+    if (x0 >= n) break;
+    __syncthreads();
+    for (int s = tid; s < k * kGsChunk; s += 256) {
+      const int c = s / kGsChunk, i = s % kGsChunk;
+      const long long x = x0 + i;
+      vs[c][i] = x < n ? V[static_cast<long long>(c) * n + x] : cmake(0.0, 0.0);
+    }
+    if (tid < kGsChunk) {
+      const long long x = x0 + tid;
+      ws[tid] = x < n ? w[x] : cmake(0.0, 0.0);
+    }
+    __syncthreads();
+    if (h) {
+      // 8 threads per element, each over k/8 columns
+      const int i = tid / 8, part = tid % 8;
+      c128 s{0.0, 0.0};
+      for (int c = part; c < k; c += 8) s = cfma(vs[c][i], hs[c], s);
+      red[tid] = s;
+      __syncthreads();
+      if (part == 0) {
+        c128 t = red[tid];
+        for (int p = 1; p < 8; ++p) t = cadd(t, red[tid + p]);
+        const c128 nw = csub(ws[i], t);
+        ws[i] = nw;
+        const long long x = x0 + i;
+        if (x < n) w[x] = nw;
+      }
+      __syncthreads();
+    }
+    if (dc < k) {
+#pragma unroll
+      for (int i = dq * 8; i < dq * 8 + 8; ++i) dot = cfmaconj(vs[dc][i], ws[i], dot);
+    }
+  }
+  __syncthreads();
+  red[tid] = dot;
+  __syncthreads();
+  if (tid < 64 && tid < k) {
+    const c128 t = cadd(cadd(red[tid], red[tid + 64]), cadd(red[tid + 128], red[tid + 192]));
+    partial[static_cast<long long>(tid) * nblk + blockIdx.x] = t;
   }
 }
 
@@ -368,7 +425,11 @@ struct BasisWs {
 };
 
 BasisWs basis_layout(char* base, long long n, int m) {
-  const long long nchunk = (n + kChunk - 1) / kChunk;
+  long long nchunk = (n + kChunk - 1) / kChunk;
+  const long long gsb = (n + kGsChunk * kGsPer - 1) / (kGsChunk * kGsPer);
+  const long long nupd = (n + kUpdChunk - 1) / kUpdChunk;
+  if (gsb > nchunk) nchunk = gsb;
+  if (nupd > nchunk) nchunk = nupd;
   size_t off = 0;
   auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~size_t(255); return o; };
   const size_t o_w0 = take(n * 16), o_w1 = take(n * 16), o_p = take(static_cast<size_t>(m) * nchunk * 16);
@@ -401,6 +462,8 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int nchunk = ceil_div(n, kChunk);
+  const int nchunk_upd = ceil_div(n, kUpdChunk);
+  const int gs_blocks = ceil_div(n, kGsChunk * kGsPer);
   c128* V = static_cast<c128*>(d_basis);
   const c128* x0 = static_cast<const c128*>(d_state);
 
@@ -425,22 +488,38 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
       break;
     }
     const int k = j + 1;
-    const dim3 gd(nchunk, ceil_div(k, kColGroup));
-    // pass 1: h1 = V^H w (alpha_j = Re h1[j]); w -= V h1
-    hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, d_alpha, j, j, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_update, dim3(nchunk), dim3(kThreads), 0, s, V, n, k, w.h, out, nullptr, w.ctl);
-    RPM_LAUNCH_CHECK();
-    // pass 2: h2 = V^H w; w -= V h2; beta = ||w||
-    hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, nullptr, 0, 0, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_update, dim3(nchunk), dim3(kThreads), 0, s, V, n, k, w.h, out, w.npart, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_finish_beta, dim3(1), dim3(kThreads), 0, s, w.npart, nchunk, d_beta, j, tol, w.ctl);
+    if (k <= kGsMaxK) {
+      // three passes over V: h1 = V^H w | w -= V h1, h2 = V^H w | w -= V h2, ||w||
+      hipLaunchKernelGGL(k_gs_pass, dim3(gs_blocks), dim3(256), 0, s, V, n, k, nullptr, out, w.partial, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, gs_blocks, w.h, d_alpha, j, j,
+                         w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_gs_pass, dim3(gs_blocks), dim3(256), 0, s, V, n, k, w.h, out, w.partial, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, gs_blocks, w.h, nullptr, 0, 0,
+                         w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, w.npart, w.ctl);
+      RPM_LAUNCH_CHECK();
+    } else {
+      const dim3 gd(nchunk, ceil_div(k, kColGroup));
+      // pass 1: h1 = V^H w (alpha_j = Re h1[j]); w -= V h1
+      hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, d_alpha, j, j, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, nullptr, w.ctl);
+      RPM_LAUNCH_CHECK();
+      // pass 2: h2 = V^H w; w -= V h2; beta = ||w||
+      hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, nullptr, 0, 0, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, w.npart, w.ctl);
+      RPM_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_finish_beta, dim3(1), dim3(kThreads), 0, s, w.npart, nchunk_upd, d_beta, j, tol, w.ctl);
     RPM_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_basis_info, dim3(1), dim3(64), 0, s, w.ctl, d_info);

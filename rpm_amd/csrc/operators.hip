@@ -200,7 +200,8 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   std::vector<c128> coefC(NC * 2 * 4);
   std::vector<int> ofsO(ngroups * KO > 0 ? ngroups * KO : 1, 0);
   std::vector<c128> coefO(ngroups * KO > 0 ? ngroups * KO * 4 : 4);
-  std::vector<unsigned char> flagO(ngroups * KO > 0 ? ngroups * KO : 1, 0);
+  std::vector<unsigned> maskO(ngroups > 0 ? ngroups : 1, 0u);
+  if (KO > 16 || KR > 16 || NC > 16) tile_ok = false;
   auto nz = [](c128 c) { return c.re != 0.0 || c.im != 0.0; };
   auto flags4 = [&](const c128* C) {
     int f = 0;
@@ -208,9 +209,43 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     if (nz(C[1]) || nz(C[2])) f |= kFlipE;
     return static_cast<unsigned char>(f);
   };
+  std::vector<int> packR(tile_ok ? NR * KR : 1, 0);
+  std::vector<c128> coefT(tile_ok ? KR * 3 * 4 : 4, c128{0, 0});
   if (tile_ok) {
     for (long long r = 0; r < NR; ++r)
       for (int k = 0; k < KR; ++k) flagR[k] |= flags4(&coefR[(r * KR + k) * 4]);
+    // right neighbours by (slot k, digit m): slot order follows the right nuclei, (d-1) slots each
+    std::vector<long long> rstride(nR);
+    {
+      long long s = 1;
+      for (int i = nR - 1; i >= 0; --i) { rstride[i] = s; s *= rdims[i]; }
+    }
+    int k0 = 0;
+    for (int i = 0; i < nR; ++i) {
+      const int d = rdims[i];
+      const c128* B = blk[nL + i];
+      for (int m = 0; m < d; ++m) {
+        int slot = 0;
+        for (int m2 = 0; m2 < d; ++m2) {
+          if (m2 == m) continue;
+          for (int e = 0; e < 2; ++e)
+            for (int e2 = 0; e2 < 2; ++e2)
+              coefT[((k0 + slot) * 3 + m) * 4 + e * 2 + e2] = B[(e * d + m) * (2 * d) + (e2 * d + m2)];
+          ++slot;
+        }
+      }
+      for (long long r = 0; r < NR; ++r) {
+        const int m = static_cast<int>((r / rstride[i]) % d);
+        int slot = 0;
+        for (int m2 = 0; m2 < d; ++m2) {
+          if (m2 == m) continue;
+          const int off = static_cast<int>((m2 - m) * rstride[i]);
+          packR[r * KR + k0 + slot] = (off & 0xffff) | (m << 16);
+          ++slot;
+        }
+      }
+      k0 += d - 1;
+    }
     for (int j = 0; j < NC; ++j) {
       const c128* B = blk[nL - 1 - j];      // d = 2, row = e*2 + m
       for (int m = 0; m < 2; ++m) {
@@ -237,7 +272,7 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
           c128* C = &coefO[(g * KO + k) * 4];
           for (int e = 0; e < 2; ++e)
             for (int e2 = 0; e2 < 2; ++e2) C[e * 2 + e2] = B[(e * d + m) * (2 * d) + (e2 * d + m2)];
-          flagO[g * KO + k] = flags4(C);
+          maskO[g] |= static_cast<unsigned>(flags4(C)) << (2 * k);
           ++k;
         }
       }
@@ -262,15 +297,16 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   const size_t o_cA = reserve(cA.size() * 16), o_cB = reserve(cB.size() * 16);
   const size_t o_offL = reserve(offL.size() * 4), o_offR = reserve(offR.size() * 4);
   const size_t o_coL = reserve(coefL.size() * 16), o_coR = reserve(coefR.size() * 16);
-  const size_t o_fR = reserve(flagR.size()), o_cC = reserve(coefC.size() * 16), o_fC = reserve(flagC.size());
-  const size_t o_oO = reserve(ofsO.size() * 4), o_cO = reserve(coefO.size() * 16), o_fO = reserve(flagO.size());
+  const size_t o_cC = reserve(coefC.size() * 16);
+  const size_t o_oO = reserve(ofsO.size() * 4), o_cO = reserve(coefO.size() * 16), o_fO = reserve(maskO.size() * 4);
+  const size_t o_pR = reserve(packR.size() * 4), o_cT = reserve(coefT.size() * 16);
   std::vector<char> host(bytes);
-  std::memcpy(host.data() + o_fR, flagR.data(), flagR.size());
   std::memcpy(host.data() + o_cC, coefC.data(), coefC.size() * 16);
-  std::memcpy(host.data() + o_fC, flagC.data(), flagC.size());
   std::memcpy(host.data() + o_oO, ofsO.data(), ofsO.size() * 4);
   std::memcpy(host.data() + o_cO, coefO.data(), coefO.size() * 16);
-  std::memcpy(host.data() + o_fO, flagO.data(), flagO.size());
+  std::memcpy(host.data() + o_fO, maskO.data(), maskO.size() * 4);
+  std::memcpy(host.data() + o_pR, packR.data(), packR.size() * 4);
+  std::memcpy(host.data() + o_cT, coefT.data(), coefT.size() * 16);
   std::memcpy(host.data() + o_dA, dA.data(), dA.size() * 16);
   std::memcpy(host.data() + o_dB, dB.data(), dB.size() * 16);
   std::memcpy(host.data() + o_cA, cA.data(), cA.size() * 16);
@@ -315,12 +351,16 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     U.cB = T.cB;
     U.offR = T.offR;
     U.coefR = T.coefR;
-    U.flagR = reinterpret_cast<const unsigned char*>(base + o_fR);
+    U.maskR = 0;
+    for (int k = 0; k < KR; ++k) U.maskR |= static_cast<unsigned>(flagR[k]) << (2 * k);
     U.coefC = reinterpret_cast<const c128*>(base + o_cC);
-    U.flagC = reinterpret_cast<const unsigned char*>(base + o_fC);
+    U.maskC = 0;
+    for (int j = 0; j < NC; ++j) U.maskC |= static_cast<unsigned>(flagC[j]) << (2 * j);
     U.ofsO = reinterpret_cast<const int*>(base + o_oO);
     U.coefO = reinterpret_cast<const c128*>(base + o_cO);
-    U.flagO = reinterpret_cast<const unsigned char*>(base + o_fO);
+    U.maskO = reinterpret_cast<const unsigned*>(base + o_fO);
+    U.packR = reinterpret_cast<const int*>(base + o_pR);
+    U.coefT = reinterpret_cast<const c128*>(base + o_cT);
     for (int e = 0; e < 4; ++e) {
       U.pdiag[e] = T.pdiag[e];
       U.pboth[e] = T.pboth[e];
@@ -474,7 +514,9 @@ int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int bat
     if (P->use_tile && P->kernel != 1) {
       if (static_cast<long long>(P->tile.ngroups) * batch > 0x7fffffffLL)
         return fail(RPM_E_INVALID, "batch %d too large for one launch", batch);
-      return launch_pair_tile(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
+      if (P->kernel == 2) return launch_pair_tile(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
+      if (P->kernel == 4) return launch_pair_tile4(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
+      return launch_pair_tile3(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
     }
     dim3 grid(ceil_div(P->t.NLR, 256), batch);
     hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, in_scale,
@@ -550,17 +592,18 @@ extern "C" int rpm_operator_apply(const rpm_operator* op, const void* d_in, void
 }
 
 extern "C" int rpm_operator_set_kernel(rpm_operator* op, int kernel) {
-  if (!op || kernel < 0 || kernel > 2) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
-  if (op->kind != 1) return kernel == 2 ? fail(RPM_E_UNSUPPORTED, "tiled kernel needs a pair operator") : RPM_OK;
+  if (!op || kernel < 0 || kernel > 4) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
+  if (op->kind != 1) return kernel >= 2 ? fail(RPM_E_UNSUPPORTED, "tiled kernels need a pair operator") : RPM_OK;
   PairOp* P = static_cast<PairOp*>(op->impl);
-  if (kernel == 2 && !P->use_tile) return fail(RPM_E_UNSUPPORTED, "the operator shape has no tiled kernel");
+  if (kernel >= 2 && !P->use_tile) return fail(RPM_E_UNSUPPORTED, "the operator shape has no tiled kernel");
   P->kernel = kernel;
   return RPM_OK;
 }
 
 extern "C" int rpm_operator_kernel(const rpm_operator* op) {
   if (!op) return -1;
-  if (op->kind != 1) return 3;
+  if (op->kind != 1) return 9;
   const PairOp* P = static_cast<const PairOp*>(op->impl);
-  return (P->use_tile && P->kernel != 1) ? 2 : 1;
+  if (!P->use_tile || P->kernel == 1) return 1;
+  return (P->kernel == 2 || P->kernel == 4) ? P->kernel : 3;
 }

@@ -118,7 +118,7 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
     for (int l = 0; l < LG; ++l) {
       const int m = (l >> j) & 1;
       const c128* C = T.coefC + (j * 2 + m) * 4;
-      mac2x2_left(C, T.flagC[j], psi[l ^ (1 << j)], acc[l]);
+      mac2x2_left(C, (T.maskC >> (2 * j)) & 3, psi[l ^ (1 << j)], acc[l]);
     }
   }
 
@@ -126,11 +126,11 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
   {
     const int* __restrict__ ofs = T.ofsO + grp * KO;
     const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
-    const unsigned char* __restrict__ fl = T.flagO + grp * KO;
+    const unsigned fm = T.maskO[grp];
 #pragma unroll 1
     for (int k = 0; k < KO; ++k) {
       const long long o = static_cast<long long>(l0 + ofs[k]) * NR + r;
-      const int f = fl[k];
+      const int f = (fm >> (2 * k)) & 3;
       const c128* C = cof + k * 4;
 #pragma unroll
       for (int l = 0; l < LG; ++l) {
@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
     c128 C[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) C[i] = T.coefR[(r * KR + k) * 4 + i];
-    const int f = T.flagR[k];
+    const int f = (T.maskR >> (2 * k)) & 3;
 #pragma unroll
     for (int l = 0; l < LG; ++l) {
       c128 q[4];
@@ -177,6 +177,251 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
 #pragma unroll
       for (int e = 0; e < 4; ++e) z[e * S + static_cast<long long>(l0 + l) * NR + r] = cscale(psi[l][e], s);
   }
+}
+
+// Variant with one (l, r) per thread: 4 outputs per thread, low register
+// pressure and high occupancy; closed left axes and right neighbours from LDS,
+// right-radical coefficients from a (slot, digit) table staged in LDS.
+template <int NR, int NC, int KO, int KR>
+__global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6, 8))) k_pair_tile3(TileTables T, const c128* __restrict__ in,
+                                                         c128* __restrict__ out, long long ld, int batch,
+                                                         const double* __restrict__ in_scale,
+                                                         c128* __restrict__ copy_out, const int* __restrict__ active) {
+  constexpr int LG = 1 << NC;
+  constexpr int NT = NR * LG;
+  if (active && *active == 0) return;
+  __shared__ c128 tile[4 * LG * NR];
+  __shared__ c128 coefT[KR * 3 * 4];
+  const int G = T.ngroups;
+  int b, grp;
+  {
+    const int g = blockIdx.x;
+    if (batch % kXcds == 0) {
+      const int x = g % kXcds, k = g / kXcds;
+      b = (k / G) * kXcds + x;
+      grp = k % G;
+    } else {
+      b = g / G;
+      grp = g % G;
+    }
+  }
+  const int tid = threadIdx.x;
+  const int l = tid / NR, r = tid - (tid / NR) * NR;
+  const long long S = T.NLR;
+  const c128* __restrict__ x = in + b * ld;
+  const int l0 = grp * LG;
+  const long long own = static_cast<long long>(l0 + l) * NR + r;
+
+  for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
+  c128 psi[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    psi[e] = x[e * S + own];
+    tile[(e * LG + l) * NR + r] = psi[e];
+  }
+
+  c128 acc[4];
+  {
+    const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
+    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
+      c128 a = cfma(d, psi[e], cmake(0.0, 0.0));
+      a = cfma(T.cA[e * T.NL + l0 + l], psi[e ^ 2], a);
+      a = cfma(T.cB[e * NR + r], psi[e ^ 1], a);
+      a = cfma(T.pboth[e], psi[e ^ 3], a);
+      acc[e] = a;
+    }
+  }
+
+  // open left axes (global / L2): offsets and coefficients uniform over the tile
+  {
+    const int* __restrict__ ofs = T.ofsO + grp * KO;
+    const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
+    const unsigned fm = T.maskO[grp];
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      const long long o = own + static_cast<long long>(ofs[k]) * NR;
+      c128 q[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) q[e] = x[e * S + o];
+      mac2x2_left(cof + k * 4, (fm >> (2 * k)) & 3, q, acc);
+    }
+  }
+
+  __syncthreads();
+  // closed left axes from LDS
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const int l2 = l ^ (1 << j);
+    const int m = (l >> j) & 1;
+    c128 q[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
+    mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, q, acc);
+  }
+  // right neighbours from LDS
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int pk = T.packR[r * KR + k];
+    const int off = static_cast<short>(pk & 0xffff);
+    const int m = pk >> 16;
+    c128 C[4], q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C[i] = coefT[(k * 3 + m) * 4 + i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l) * NR + r + off];
+    mac2x2_right(C, (T.maskR >> (2 * k)) & 3, q, acc);
+  }
+
+  const double s = in_scale ? *in_scale : 1.0;
+  c128* __restrict__ y = out + b * ld;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    __builtin_nontemporal_store(acc[e].re * s, &y[e * S + own].re);
+    __builtin_nontemporal_store(acc[e].im * s, &y[e * S + own].im);
+  }
+  if (copy_out) {
+    c128* z = copy_out + b * ld;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(psi[e], s);
+  }
+}
+
+// Variant of k_pair_tile3 that issues every global load of a thread (its own
+// 4 amplitudes and all open-axis neighbours) before any arithmetic, trading
+// occupancy for memory-level parallelism; the left index is made wave-uniform
+// so diagonal / flip / closed-axis coefficients come through scalar loads.
+template <int NR, int NC, int KO, int KR>
+__global__ void __launch_bounds__(NR << NC) k_pair_tile4(TileTables T, const c128* __restrict__ in,
+                                                         c128* __restrict__ out, long long ld, int batch,
+                                                         const double* __restrict__ in_scale,
+                                                         c128* __restrict__ copy_out, const int* __restrict__ active) {
+  constexpr int LG = 1 << NC;
+  constexpr int NT = NR * LG;
+  if (active && *active == 0) return;
+  __shared__ c128 tile[4 * LG * NR];
+  __shared__ c128 coefT[KR * 3 * 4];
+  const int G = T.ngroups;
+  int b, grp;
+  {
+    const int g = blockIdx.x;
+    if (batch % kXcds == 0) {
+      const int x = g % kXcds, k = g / kXcds;
+      b = (k / G) * kXcds + x;
+      grp = k % G;
+    } else {
+      b = g / G;
+      grp = g % G;
+    }
+  }
+  const int tid = threadIdx.x;
+  // NR is a multiple of the wave size, so l is uniform across the wave
+  const int l = __builtin_amdgcn_readfirstlane(tid / NR);
+  const int r = tid - l * NR;
+  const int S = static_cast<int>(T.NLR);
+  const c128* __restrict__ x = in + b * ld;
+  const int l0 = grp * LG;
+  const int own = (l0 + l) * NR + r;
+  const int* __restrict__ ofs = T.ofsO + grp * KO;
+
+  c128 psi[4];
+  c128 q[KO][4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) psi[e] = x[e * S + own];
+#pragma unroll
+  for (int k = 0; k < KO; ++k) {
+    const int o = own + ofs[k] * NR;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q[k][e] = x[e * S + o];
+  }
+  for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) tile[(e * LG + l) * NR + r] = psi[e];
+
+  c128 acc[4];
+  {
+    const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
+    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
+      c128 a = cfma(d, psi[e], cmake(0.0, 0.0));
+      a = cfma(T.cA[e * T.NL + l0 + l], psi[e ^ 2], a);
+      a = cfma(T.cB[e * NR + r], psi[e ^ 1], a);
+      a = cfma(T.pboth[e], psi[e ^ 3], a);
+      acc[e] = a;
+    }
+  }
+  {
+    const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
+    const unsigned fm = T.maskO[grp];
+#pragma unroll
+    for (int k = 0; k < KO; ++k) mac2x2_left(cof + k * 4, (fm >> (2 * k)) & 3, q[k], acc);
+  }
+
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < NC; ++j) {
+    const int l2 = l ^ (1 << j);
+    const int m = (l >> j) & 1;
+    c128 qq[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) qq[e] = tile[(e * LG + l2) * NR + r];
+    mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, qq, acc);
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int pk = T.packR[r * KR + k];
+    const int off = static_cast<short>(pk & 0xffff);
+    const int m = pk >> 16;
+    c128 C[4], qq[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C[i] = coefT[(k * 3 + m) * 4 + i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) qq[e] = tile[(e * LG + l) * NR + r + off];
+    mac2x2_right(C, (T.maskR >> (2 * k)) & 3, qq, acc);
+  }
+
+  const double s = in_scale ? *in_scale : 1.0;
+  c128* __restrict__ y = out + b * ld;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    __builtin_nontemporal_store(acc[e].re * s, &y[e * S + own].re);
+    __builtin_nontemporal_store(acc[e].im * s, &y[e * S + own].im);
+  }
+  if (copy_out) {
+    c128* z = copy_out + b * ld;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(psi[e], s);
+  }
+}
+
+int launch_pair_tile4(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                      c128* copy_out, const int* active, hipStream_t stream) {
+  const int blocks = T.ngroups * batch;
+  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
+    hipLaunchKernelGGL((k_pair_tile4<192, 2, 7, 8>), dim3(blocks), dim3(192 * 4), 0, stream, T, in, out, ld, batch,
+                       in_scale, copy_out, active);
+  } else {
+    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  }
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                      c128* copy_out, const int* active, hipStream_t stream) {
+  const int blocks = T.ngroups * batch;
+  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
+    hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8>), dim3(blocks), dim3(192 * 4), 0, stream, T, in, out, ld, batch,
+                       in_scale, copy_out, active);
+  } else {
+    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  }
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
 }
 
 int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,

@@ -18,12 +18,14 @@ struct TileTables {
   const c128* cB;                  // [4][NR]
   const int* offR;                 // [NR][KR]
   const c128* coefR;               // [NR][KR][4]
-  const unsigned char* flagR;      // [KR]
+  unsigned maskR;                  // 2 bits per right slot k: bit 2k same-electron, 2k+1 electron-flip
   const c128* coefC;               // [NC][2][4]   closed left axes (spin-1/2), j = 0 is the last left axis
-  const unsigned char* flagC;      // [NC]
+  unsigned maskC;                  // 2 bits per closed axis j
   const int* ofsO;                 // [ngroups][KO]  left-index offset of each open-axis neighbour
   const c128* coefO;               // [ngroups][KO][4]
-  const unsigned char* flagO;      // [ngroups][KO]
+  const unsigned* maskO;           // [ngroups]  2 bits per open neighbour k
+  const int* packR;                // [NR][KR]  (r2 - r) & 0xffff | m << 16 : neighbour offset and digit
+  const c128* coefT;               // [KR][3][4] right coefficients by (slot, digit m)
   c128 pdiag[4];
   c128 pboth[4];
 };
@@ -31,5 +33,9 @@ struct TileTables {
 int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
                      c128* copy_out, const int* active, hipStream_t stream);
 bool pair_tile_supported(int NR, int NC, int KO, int KR);
+int launch_pair_tile4(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                      c128* copy_out, const int* active, hipStream_t stream);
+int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
+                      c128* copy_out, const int* active, hipStream_t stream);
 
 }  // namespace rpm

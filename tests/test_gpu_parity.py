@@ -302,19 +302,22 @@ def test_full_system_short_horizon_matches_oracle():
     assert _rel(a.final_state, b.final_state) < 1e-10
 
 
+@pytest.mark.parametrize("kernel", [2, 3, 4])
 @pytest.mark.parametrize("batch", [1, 3, 8, 16])
-def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch):
+def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch, kernel):
     h, o = production_pair
     dev = h._device_op()
-    assert dev.kernel == 2, "production operator should use the tiled pair kernel"
+    assert dev.kernel in (2, 3), "production operator should use a tiled pair kernel"
     n = h.dimension
     v = workloads.normal_vectors(n, batch, seed=10 + batch)
     x = torch.from_numpy(v).cuda()
     y_tile = torch.empty_like(x)
     y_gather = torch.empty_like(x)
-    dev.apply(x, y_tile, batch=batch, ld=n)
-    dev.set_kernel(1)
     try:
+        dev.set_kernel(kernel)
+        assert dev.kernel == kernel
+        dev.apply(x, y_tile, batch=batch, ld=n)
+        dev.set_kernel(1)
         assert dev.kernel == 1
         dev.apply(x, y_gather, batch=batch, ld=n)
     finally:
@@ -330,7 +333,7 @@ def test_tiled_kernel_random_tensor_system():
     sysr = workloads.full_random_system(seed=7)
     h = sysr.hamiltonian(theta_deg=63.0, phi_deg=20.0)
     o = sysr.hamiltonian(theta_deg=63.0, phi_deg=20.0, builder=ocore.build_hamiltonian)
-    assert h._device_op().kernel == 2
+    assert h._device_op().kernel in (2, 3)
     v = workloads.normal_vectors(h.dimension, 1, seed=8)[0]
     assert _rel(h.matvec(v), o.matvec(v)) <= HPSI_TOL
 
