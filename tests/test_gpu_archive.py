@@ -1,0 +1,62 @@
+"""End-to-end: GPU trajectories against the reference's archived full-system yields.
+
+results/quadrupole/full14_uncoupled_tau5_29angles/raw_sample_yields.csv of the
+reference (fixture copy: tests/golden/archived_full14_yields.csv) holds 464
+per-trajectory singlet yields at dim 221,184 (50 uT, tau = 5 us, tail bound
+1e-5, horizon 57.5646 us, 46,053-point grid, production Krylov rules, budget
+2.4e-5, seed 20260731).  Each archived row took 3.4-7.8 CPU hours.  Two rows
+are re-run here from scratch on the GPU; the yield must agree within the
+archive's own propagation estimate (~1e-6), and the fine/coarse quadrature
+difference within 1e-9.
+"""
+
+import csv
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from rpm_amd import core  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "archived_full14_yields.csv")
+
+
+def _rows():
+    with open(GOLDEN) as f:
+        return list(csv.DictReader(f))
+
+
+def _pick(rows, interactions, scale, theta, sample):
+    for r in rows:
+        if (r["electron_interactions"] == interactions and float(r["scale"]) == scale
+                and float(r["theta_deg"]) == theta and int(r["sample_id"]) == sample):
+            return r
+    raise KeyError((interactions, scale, theta, sample))
+
+
+@pytest.mark.parametrize("interactions,scale,theta,sample", [
+    ("D=J=0", 0.0, 0.0, 0),
+    ("D/J=-3", 1.0, 90.0, 1),
+])
+def test_archived_full_system_yield(interactions, scale, theta, sample):
+    row = _pick(_rows(), interactions, scale, theta, sample)
+    model = core.ProductionModel(quadrupole_scale=scale,
+                                 ratio_D_over_J=None if interactions == "D=J=0" else -3.0)
+    h = core.build_hamiltonian(theta, 0.05, model=model)
+    state = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, sample)
+    assert core.initial_state_sha256(state) == row["initial_state_sha256"]
+    cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
+                           state_error_budget=2.4e-5, quadrupole_scale=scale)
+    out = core.propagate_trajectory(h, state, cfg)
+    ref = float(row["yield"])
+    print(f"{interactions} Q={scale} theta={theta} s={sample}: GPU {out['yield']:.12f} archive {ref:.12f} "
+          f"diff {out['yield'] - ref:.2e}; quad err {out['quadrature_error']:.3e} vs {float(row['quadrature_error']):.3e}; "
+          f"prop est {out['yield_propagation_estimate']:.3e} vs {float(row['yield_propagation_estimate']):.3e}; "
+          f"steps {out['result'].stats.accepted_steps}/{out['result'].stats.rejected_steps} "
+          f"matvecs {out['result'].stats.matvecs}", flush=True)
+    assert abs(out["yield"] - ref) < max(2e-6, float(row["yield_propagation_estimate"]))
+    assert abs(out["quadrature_error"] - float(row["quadrature_error"])) < 1e-9
+    assert out["yield_propagation_estimate"] < 5e-6
