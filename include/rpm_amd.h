@@ -22,6 +22,16 @@
  *   rpm_krylov_combine         endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
  *   rpm_norm                   np.linalg.norm(state)                   R/rpm_statevector_engine.py:656,774,909
  *   rpm_singlet_population     singlet_population                      R/rpm_statevector_engine.py:492-497
+ *
+ * Batched entry points (many trajectories per launch; each replaces the same
+ * reference function as its single-trajectory form above, called once per
+ * trajectory by the drivers R/rpm_full14_quadrupole_experiment.py:154-194 and
+ * R/rpm_quantum_needle_jd_sweeps.py:466-546):
+ *   rpm_operator_apply_pairs   .matvec with a per-vector electron 4x4 (field orientation/strength)
+ *   rpm_lanczos_basis_batch    AdaptiveLanczosPropagator._basis        R/rpm_statevector_engine.py:651-688
+ *   rpm_krylov_tasks           _candidate / evaluate_projected         R/rpm_statevector_engine.py:690-715,531-538
+ *   rpm_singlet_gram_batch     SingletPopulationObservable.project_basis R/rpm_statevector_engine.py:521-529
+ *   rpm_krylov_combine_batch   endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
  */
 #ifndef RPM_AMD_H
 #define RPM_AMD_H
@@ -84,13 +94,20 @@ int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, in
                        long long ld, void* stream);
 
 /*
+ * d_out[b] = H_b d_in[b] where H_b is the pair operator with its electron 4x4
+ * (electron_pair of rpm_operator_create_pair) replaced by d_pairs[b]
+ * (16 complex per vector, row-major, row = eA*2 + eB).  Pair operators only.
+ */
+int rpm_operator_apply_pairs(const rpm_operator* op, const void* d_in, void* d_out, int batch,
+                             long long ld, const void* d_pairs, void* stream);
+
+/*
  * H·psi kernel selection (no counterpart in the reference; for tests and
- * benchmarks): 0 automatic (a tiled pair kernel when the shape allows it),
- * 1 the per-amplitude gather kernel, 2 the register-tiled kernel, 3 the
- * LDS-tiled kernel, 4 the LDS-tiled kernel with all global loads issued up
- * front (2-4 give RPM_E_UNSUPPORTED when the operator shape has no tiled
- * form).  rpm_operator_kernel reports the kernel the next apply will use:
- * 1-4 as above, 9 the generic local-term kernel.
+ * benchmarks): 0 automatic (the LDS-tiled pair kernel when the shape allows
+ * it), 1 the per-amplitude gather kernel, 2 the register-tiled kernel, 3 the
+ * LDS-tiled kernel (2-3 give RPM_E_UNSUPPORTED when the operator shape has no
+ * tiled form).  rpm_operator_kernel reports the kernel the next apply will
+ * use: 1-3 as above, 9 the generic local-term kernel.
  */
 int rpm_operator_set_kernel(rpm_operator* op, int kernel);
 int rpm_operator_kernel(const rpm_operator* op);
@@ -110,6 +127,49 @@ int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, v
                       double* d_alpha, double* d_beta, int* d_info, double* d_norm,
                       double breakdown_tolerance, void* d_workspace, size_t workspace_bytes,
                       void* stream);
+
+/*
+ * Batched CGS2 basis: trajectory b < batch starts from d_states + b*ld (ld == N) and
+ * builds min(d_targets[b], max_target) vectors into d_basis + b*m*N (vector j
+ * at + j*N).  d_targets[b] <= 0 leaves trajectory b untouched (its basis,
+ * tridiagonal, norm and info keep their values).  d_pairs (nullable): per
+ * trajectory electron 4x4 as in rpm_operator_apply_pairs.  d_alpha, d_beta:
+ * batch x m (row b at b*m); d_info: batch x 4 (info[4b] = size built,
+ * info[4b+1] = 1 if no breakdown); d_norm[batch].  max_target <= m <= 64.
+ */
+size_t rpm_lanczos_batch_workspace_bytes(const rpm_operator* op, int m, int batch);
+int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_states, long long ld,
+                            const int* d_targets, int max_target, const void* d_pairs, void* d_basis,
+                            int m, double* d_alpha, double* d_beta, int* d_info, double* d_norm,
+                            double breakdown_tolerance, void* d_workspace, size_t workspace_bytes,
+                            void* stream);
+
+/*
+ * Batched small-matrix evaluations.  d_tasks: ntasks records of 32 bytes
+ *   { int32 b, dim, low, out; float64 t; int32 kind, pad; }
+ * evaluating c = exp(-i T_dim t) e_1 of trajectory b's tridiagonal
+ * (d_alpha/d_beta + b*ab_stride) with dim <= that trajectory's basis size.
+ *   kind 0 (candidate): d_v0[out] = safety*norm[b]*||c - pad(c_low)|| (0 if low <= 0 or low >= dim)
+ *   kind 1 (output):    d_v0[out] = norm[b]^2 Re(c^H P_b c) with P_b = d_P + b*p_stride
+ *                       (dim x dim row-major), d_v1[out] = the point error as above
+ * d_coeffs (nullable): row out (m_stride complex) receives c padded with zeros.
+ */
+int rpm_krylov_tasks(const void* d_tasks, int ntasks, const double* d_alpha, const double* d_beta,
+                     int ab_stride, const double* d_norm, const void* d_P, long long p_stride,
+                     double safety, double* d_v0, double* d_v1, void* d_coeffs, int m_stride,
+                     void* stream);
+
+/* Batched rpm_singlet_gram: P_b (k_b x k_b at d_P + b*m*m) of the first k_b = d_k[b]
+ * vectors of basis b (d_basis + b*m*N); k_b <= 0 skips b. */
+size_t rpm_singlet_gram_batch_workspace_bytes(long long nn, int batch);
+int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k, int batch, long long nn, void* d_P,
+                           void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* Batched rpm_krylov_combine: d_out + b*out_ld = scale[b] * sum_{j<k_b} basis_b[j] c_b[j],
+ * basis_b = d_basis + b*m*n, c_b = d_c + b*c_stride, k_b = d_k[b] (<= 0 skips b). */
+int rpm_krylov_combine_batch(const void* d_basis, int m, const int* d_k, int batch, long long n,
+                             const void* d_c, int c_stride, const double* d_scale, void* d_out,
+                             long long out_ld, void* stream);
 
 /*
  * Embedded error estimates for a ladder of candidate dimensions.

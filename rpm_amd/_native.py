@@ -54,6 +54,16 @@ SIGNATURES = {
     "rpm_norm": (ctypes.c_int, [_vp, _ll, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),
     "rpm_singlet_population": (ctypes.c_int, [_vp, _ll, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),
     "rpm_reduce_workspace_bytes": (_sz, [_ll, ctypes.c_int]),
+    "rpm_operator_apply_pairs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, _ll, _vp, _vp]),
+    "rpm_lanczos_batch_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
+    "rpm_lanczos_basis_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _ll, _vp, ctypes.c_int, _vp, _vp,
+                                               ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_double, _vp, _sz, _vp]),
+    "rpm_krylov_tasks": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp, _ll,
+                                        ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, _vp]),
+    "rpm_singlet_gram_batch_workspace_bytes": (_sz, [_ll, ctypes.c_int]),
+    "rpm_singlet_gram_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _ll, _vp, _vp, _sz, _vp]),
+    "rpm_krylov_combine_batch": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _ll, _vp, ctypes.c_int,
+                                                _vp, _vp, _ll, _vp]),
 }
 
 _lib = None

@@ -1,0 +1,418 @@
+"""Batched trajectories: many random-phase states x field orientations/strengths.
+
+Each trajectory follows exactly the step rules of
+``AdaptiveLanczosPropagator.propagate`` (R/rpm_statevector_engine.py:717-933,
+singlet-projected path 735-741, 861-902): the rules run as one small host
+coroutine per trajectory, and every device operation they ask for (Lanczos
+basis, candidate errors, projected outputs, endpoint update) is served for
+all trajectories that want it in one batched launch:
+
+* ``rpm_lanczos_basis_batch`` -- CGS2 bases of all trajectories that need one,
+  each with its own target size, breakdown flag and electron 4x4 (so one
+  launch mixes field orientations and strengths);
+* ``rpm_krylov_tasks`` -- every candidate and output-time evaluation of the
+  round in one launch;
+* ``rpm_singlet_gram_batch`` / ``rpm_krylov_combine_batch`` -- projections and
+  endpoint updates.
+
+A round serves the phases in the order basis -> candidates -> outputs ->
+update, so one host synchronisation per phase covers the whole batch.  The
+per-trajectory arithmetic is that of the single-trajectory path (same kernels,
+same reduction order), so a batched trajectory reproduces its plain run.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import Callable, Iterable, Sequence
+
+import numpy as np
+
+from . import _native as nat
+from .engine import (
+    CompiledFullPairHamiltonian, PropagationResult, PropagationStats, _pair_form,
+    state_error_to_population_bound,
+)
+
+_TASK = np.dtype([("b", "<i4"), ("dim", "<i4"), ("low", "<i4"), ("out", "<i4"), ("t", "<f8"), ("kind", "<i4"),
+                  ("pad", "<i4")])
+assert _TASK.itemsize == 32
+
+_BASIS, _CAND, _OUT, _ADV, _INIT = "basis", "cand", "out", "adv", "init"
+_PHASES = (_INIT, _BASIS, _CAND, _OUT, _ADV)
+
+
+def electron_pair_matrix(hamiltonian) -> np.ndarray:
+    """The 4x4 electron matrix (Zeeman + electron coupling) of a pair-form Hamiltonian, row = eA*2 + eB."""
+    form = _pair_form(hamiltonian.dimensions, hamiltonian.terms,
+                      getattr(hamiltonian, "left_axes", None), getattr(hamiltonian, "right_axes", None))
+    if form is None:
+        raise ValueError("The Hamiltonian has no pair-stencil form.")
+    return form[3]
+
+
+def pair_matrices(base, hamiltonians: Sequence) -> np.ndarray:
+    """Electron 4x4 of each Hamiltonian, checking that only that part differs from ``base``.
+
+    The batched operator shares the nuclear blocks of ``base``; a Hamiltonian
+    whose nuclear part differs cannot ride on it and raises ValueError.
+    """
+    ref = _pair_form(base.dimensions, base.terms, base.left_axes, base.right_axes)
+    out = np.empty((len(hamiltonians), 4, 4), dtype=complex)
+    for i, h in enumerate(hamiltonians):
+        if tuple(h.dimensions) != tuple(base.dimensions):
+            raise ValueError("All Hamiltonians of a batch must share the dimensions.")
+        form = _pair_form(h.dimensions, h.terms, h.left_axes, h.right_axes)
+        if form is None or len(form[2]) != len(ref[2]) or any(
+                not np.array_equal(a, b) for a, b in zip(form[2], ref[2])):
+            raise ValueError("Batched Hamiltonians may differ only in their electron terms (Zeeman, coupling).")
+        out[i] = form[3]
+    return out
+
+
+class BatchedLanczosPropagator:
+    """Adaptive Lanczos propagation of a batch of trajectories with the singlet observable.
+
+    Parameters as ``AdaptiveLanczosPropagator`` (R/rpm_statevector_engine.py:620-649).
+    ``hamiltonian`` must have the pair-stencil form; per-trajectory field
+    orientations/strengths enter as electron 4x4 matrices (``pairs``).
+    """
+
+    def __init__(self, hamiltonian: CompiledFullPairHamiltonian, *, minimum_dimension: int = 24,
+                 maximum_dimension: int = 64, dimension_increment: int = 8, embedded_offset: int = 4,
+                 safety_factor: float = 2.0, state_error_budget: float = 2.4e-5, initial_step_us: float = 0.05,
+                 maximum_step_us: float = 0.25, minimum_step_us: float = 1e-9,
+                 breakdown_tolerance: float = 1e-14) -> None:
+        if minimum_dimension <= embedded_offset:
+            raise ValueError("minimum_dimension must exceed embedded_offset.")
+        if maximum_dimension < minimum_dimension:
+            raise ValueError("maximum_dimension must not be smaller than minimum_dimension.")
+        if not getattr(hamiltonian, "uses_pair_stencil", False):
+            raise ValueError("Batched propagation needs a Hamiltonian with the pair-stencil form.")
+        self.hamiltonian = hamiltonian
+        self.minimum_dimension = minimum_dimension
+        self.maximum_dimension = maximum_dimension
+        self.dimension_increment = dimension_increment
+        self.embedded_offset = embedded_offset
+        self.safety_factor = safety_factor
+        self.state_error_budget = state_error_budget
+        self.initial_step_us = initial_step_us
+        self.maximum_step_us = maximum_step_us
+        self.minimum_step_us = minimum_step_us
+        self.breakdown_tolerance = breakdown_tolerance
+        self.rounds = 0
+
+    @classmethod
+    def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
+        """A batched propagator with the settings of a single-trajectory one."""
+        return cls(propagator.hamiltonian if hamiltonian is None else hamiltonian,
+                   minimum_dimension=propagator.minimum_dimension, maximum_dimension=propagator.maximum_dimension,
+                   dimension_increment=propagator.dimension_increment, embedded_offset=propagator.embedded_offset,
+                   safety_factor=propagator.safety_factor, state_error_budget=propagator.state_error_budget,
+                   initial_step_us=propagator.initial_step_us, maximum_step_us=propagator.maximum_step_us,
+                   minimum_step_us=propagator.minimum_step_us, breakdown_tolerance=propagator.breakdown_tolerance)
+
+    def _ladder(self, start):
+        c = list(range(start, self.maximum_dimension + 1, self.dimension_increment))
+        if not c or c[-1] != self.maximum_dimension:
+            c.append(self.maximum_dimension)
+        return c
+
+    # -- per-trajectory rules (R/rpm_statevector_engine.py:717-933) ----------
+
+    def _rules(self, times: np.ndarray, replay, callback):
+        """Coroutine: yields (phase, args) requests, receives their results, returns the result tuple."""
+        nt = times.size
+        values = np.empty(nt)
+        bounds = np.empty(nt)
+        errors = np.empty(nt)
+        cursor, now, acc_err = 0, 0.0, 0.0
+        accepted = rejected = max_dim = matvecs = 0
+        used = []
+        record = []
+        max_norm_err, pop0 = yield (_INIT, None)
+        if times[0] == 0.0:
+            values[0] = pop0
+            bounds[0] = errors[0] = 0.0
+            cursor = 1
+        t_final = float(times[-1])
+        step = min(self.initial_step_us, t_final) if t_final else 0.0
+        prefer_max = False
+        mn, mx, inc, off = self.minimum_dimension, self.maximum_dimension, self.dimension_increment, \
+            self.embedded_offset
+
+        while cursor < nt:
+            trial = min(step, t_final - now)
+            if trial < self.minimum_step_us:
+                raise RuntimeError("Lanczos step fell below the configured minimum.")
+            if replay is not None:
+                if len(record) >= len(replay):
+                    raise RuntimeError("Replayed schedule ended before the final time.")
+                rec = replay[len(record)]
+
+                def ok(dim, e, rec=rec):
+                    return bool(rec[1]) and int(dim) == int(rec[2])
+            else:
+                budget_now = self.state_error_budget * trial / max(t_final, trial)
+
+                def ok(dim, e, b=budget_now):
+                    return e <= b
+            target = mx if prefer_max else mn
+            size, snorm = yield (_BASIS, target)
+            matvecs += size
+            if snorm == 0.0:
+                raise ValueError("Cannot propagate the zero vector.")
+            sizes = [size]
+            budget = self.state_error_budget * trial / max(t_final, trial)
+            selected, row, err = None, None, math.inf
+            exact = size < target
+
+            def search(ladder):
+                errs, rows = yield (_CAND, (ladder, off, trial))
+                e = math.inf
+                for i, dim in enumerate(ladder):
+                    e = float(errs[i])
+                    if ok(dim, e):
+                        return dim, rows[i], e
+                return None, None, e
+
+            if exact:
+                _, rows = yield (_CAND, ([size], 0, trial))
+                selected, row, err = size, rows[0], 0.0
+            elif not prefer_max:
+                errs, rows = yield (_CAND, ([mn], off, trial))
+                err = float(errs[0])
+                if ok(mn, err):
+                    selected, row = mn, rows[0]
+                elif mx > mn:
+                    size, snorm = yield (_BASIS, mx)
+                    matvecs += size
+                    sizes.append(size)
+                    prefer_max = True
+                    exact = size < mx
+                    if exact:
+                        _, rows = yield (_CAND, ([size], 0, trial))
+                        selected, row, err = size, rows[0], 0.0
+                    else:
+                        selected, row, err = yield from search(self._ladder(mn + inc))
+            else:
+                selected, row, err = yield from search(self._ladder(mn))
+            if selected is None or (replay is None and err > budget):
+                rejected += 1
+                record.append((trial, False, 0, tuple(sizes)))
+                step = 0.5 * trial
+                continue
+
+            record.append((trial, True, int(selected), tuple(sizes)))
+            max_dim = max(max_dim, selected)
+            if selected > mn:
+                prefer_max = True
+            low = max(1, selected - off)
+            low_arg = 0 if (exact or low == selected) else low
+            step_end = now + trial
+            stop = cursor
+            while stop < nt and times[stop] <= step_end + 1e-13:
+                stop += 1
+            if stop > cursor:
+                vals, perr = yield (_OUT, (selected, low_arg, times[cursor:stop] - now))
+                values[cursor:stop] = vals
+                errors[cursor:stop] = acc_err + perr
+                bounds[cursor:stop] = [state_error_to_population_bound(e) for e in errors[cursor:stop]]
+                cursor = stop
+            norm_err = yield (_ADV, (row, selected))
+            now = step_end
+            acc_err += err
+            accepted += 1
+            used.append(trial)
+            max_norm_err = max(max_norm_err, norm_err)
+            if callback is not None:
+                callback(now, accepted, rejected, matvecs, acc_err)
+            step = min(self.maximum_step_us, 1.5 * trial) if err < 0.1 * budget else trial
+
+        stats = PropagationStats(accepted, rejected, matvecs, max_dim, min(used) if used else 0.0,
+                                 max(used) if used else 0.0, acc_err, max_norm_err)
+        return values, bounds, errors, stats, tuple(record)
+
+    # -- driver -------------------------------------------------------------
+
+    def propagate(self, initial_states, times_us: Iterable[float], *, pairs=None, schedules=None,
+                  progress_callback: Callable | None = None) -> list[PropagationResult]:
+        """Propagate B trajectories; returns one ``PropagationResult`` per trajectory.
+
+        ``initial_states``: (B, N) NumPy array or complex128 GPU tensor (copied).
+        ``pairs``: optional (B, 4, 4) electron matrices (see ``pair_matrices``);
+        default: the Hamiltonian's own.  ``schedules``: optional per-trajectory
+        recorded schedules to replay (entries may be None).
+        ``progress_callback(b, now, accepted, rejected, matvecs, acc_err)``
+        runs after each accepted step of trajectory b.
+        """
+        torch = nat.require_gpu()
+        lib = nat.load()
+        dev = nat.device()
+        h = self.hamiltonian
+        op = h._device_op()
+        n = h.dimension
+        times = np.asarray(tuple(float(t) for t in times_us), dtype=float)
+        if times.ndim != 1 or times.size == 0:
+            raise ValueError("At least one observation time is required.")
+        if np.any(times < 0.0) or np.any(np.diff(times) <= 0.0):
+            raise ValueError("Observation times must be nonnegative and strictly increasing.")
+        x0 = initial_states
+        if nat.is_torch(x0):
+            states = x0.to(device=dev, dtype=torch.complex128).reshape(x0.shape[0], -1).clone()
+        else:
+            arr = np.asarray(x0, dtype=complex)
+            states = torch.from_numpy(np.ascontiguousarray(arr.reshape(arr.shape[0], -1))).to(dev)
+        B = states.shape[0]
+        if states.shape[1] != n:
+            raise ValueError("Initial state dimension does not match the Hamiltonian.")
+        if B < 1 or B > 65535:
+            raise ValueError("batch size must be in 1..65535")
+        m = min(self.maximum_dimension, n)
+        if m > 64:
+            raise ValueError("maximum_dimension above 64 is not supported by the device kernels.")
+        d_pairs = None
+        if pairs is not None:
+            p = np.asarray(pairs, dtype=complex)
+            if p.shape != (B, 4, 4):
+                raise ValueError(f"pairs must have shape ({B}, 4, 4)")
+            if not np.allclose(p, np.conj(np.transpose(p, (0, 2, 1))), atol=1e-12 * max(1.0, np.abs(p).max())):
+                raise ValueError("Electron matrices must be Hermitian.")
+            d_pairs = torch.from_numpy(np.ascontiguousarray(p.reshape(B, 16))).to(dev)
+        if schedules is not None and len(schedules) != B:
+            raise ValueError("schedules must have one entry per trajectory")
+
+        V = torch.empty((B, m, n), dtype=torch.complex128, device=dev)
+        alpha = torch.zeros((B, m), dtype=torch.float64, device=dev)
+        beta = torch.zeros((B, m), dtype=torch.float64, device=dev)
+        info = torch.zeros((B, 4), dtype=torch.int32, device=dev)
+        norm = torch.zeros(B, dtype=torch.float64, device=dev)
+        ws = nat.workspace(lib.rpm_lanczos_batch_workspace_bytes(ctypes.c_void_p(op.handle), m, B))
+        gram_ws = nat.workspace(lib.rpm_singlet_gram_batch_workspace_bytes(n // 4, B))
+        P = torch.zeros((B, m, m), dtype=torch.complex128, device=dev)
+        ladder_len = len(self._ladder(self.minimum_dimension))
+        coeffs = torch.zeros((B * ladder_len, m), dtype=torch.complex128, device=dev)
+        cand_err = torch.zeros(B * ladder_len, dtype=torch.float64, device=dev)
+        size_h = np.zeros(B, dtype=np.int64)
+        red_ws = nat.workspace(lib.rpm_reduce_workspace_bytes(n, B))
+        nrm_out = torch.empty(B, dtype=torch.float64, device=dev)
+        pop_out = torch.empty(B, dtype=torch.float64, device=dev)
+        sp = ctypes.c_void_p(nat.stream_ptr())
+        vp = ctypes.c_void_p
+
+        def norms():
+            nat.check(lib.rpm_norm(vp(nat.ptr(states)), n, B, n, vp(nat.ptr(nrm_out)), vp(nat.ptr(red_ws)),
+                                   red_ws.numel(), sp))
+            return nrm_out.cpu().numpy()
+
+        def run_tasks(tasks, with_p, v0, v1, coeff_buf):
+            dt = torch.from_numpy(tasks.view(np.uint8)).to(dev)
+            nat.check(lib.rpm_krylov_tasks(
+                vp(nat.ptr(dt)), len(tasks), vp(nat.ptr(alpha)), vp(nat.ptr(beta)), m, vp(nat.ptr(norm)),
+                vp(nat.ptr(P) if with_p else None), m * m, float(self.safety_factor), vp(nat.ptr(v0)),
+                vp(nat.ptr(v1) if v1 is not None else None),
+                vp(nat.ptr(coeff_buf) if coeff_buf is not None else None), m, sp))
+
+        cb = progress_callback
+        gens, pending, results = {}, {}, {}
+        for b in range(B):
+            replay = None if schedules is None or schedules[b] is None else list(schedules[b])
+            call = None if cb is None else (lambda *a, b=b: cb(b, *a))
+            gens[b] = self._rules(times, replay, call)
+            pending[b] = next(gens[b])
+
+        def resume(b, value):
+            try:
+                pending[b] = gens[b].send(value)
+            except StopIteration as stop:
+                results[b] = stop.value
+                del pending[b]
+
+        self.rounds = 0
+        while pending:
+            self.rounds += 1
+            for phase in _PHASES:
+                group = sorted(b for b, (ph, _) in pending.items() if ph == phase)
+                if not group:
+                    continue
+                if phase == _INIT:
+                    nv = norms()
+                    nat.check(lib.rpm_singlet_population(vp(nat.ptr(states)), n // 4, B, n, vp(nat.ptr(pop_out)),
+                                                         vp(nat.ptr(red_ws)), red_ws.numel(), sp))
+                    pv = pop_out.cpu().numpy()
+                    for b in group:
+                        resume(b, (abs(float(nv[b]) - 1.0), float(pv[b])))
+                elif phase == _BASIS:
+                    tg = np.zeros(B, dtype=np.int32)
+                    for b in group:
+                        tg[b] = min(int(pending[b][1]), n)
+                    d_tg = torch.from_numpy(tg).to(dev)
+                    nat.check(lib.rpm_lanczos_basis_batch(
+                        vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_tg)), int(tg.max()),
+                        vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)), m,
+                        vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
+                        float(self.breakdown_tolerance), vp(nat.ptr(ws)), ws.numel(), sp))
+                    inf_h = info.cpu().numpy()
+                    nrm_h = norm.cpu().numpy()
+                    for b in group:
+                        size_h[b] = int(inf_h[b, 0])
+                        resume(b, (int(inf_h[b, 0]), float(nrm_h[b])))
+                elif phase == _CAND:
+                    tasks, spans = [], {}
+                    for b in group:
+                        dims, offset, t = pending[b][1]
+                        start = len(tasks)
+                        for d in dims:
+                            dim = min(int(d), int(size_h[b]))
+                            tasks.append((b, dim, max(1, dim - offset), len(tasks), t, 0, 0))
+                        spans[b] = (start, len(tasks))
+                    ta = np.array(tasks, dtype=_TASK)
+                    run_tasks(ta, False, cand_err, None, coeffs)
+                    eh = cand_err[:len(ta)].cpu().numpy()
+                    for b in group:
+                        s0, s1 = spans[b]
+                        resume(b, (eh[s0:s1], list(range(s0, s1))))
+                elif phase == _OUT:
+                    kb = np.zeros(B, dtype=np.int32)
+                    tasks, spans = [], {}
+                    for b in group:
+                        sel, low_arg, offs = pending[b][1]
+                        kb[b] = sel
+                        start = len(tasks)
+                        for t in offs:
+                            tasks.append((b, sel, low_arg, len(tasks), float(t), 1, 0))
+                        spans[b] = (start, len(tasks))
+                    d_kb = torch.from_numpy(kb).to(dev)
+                    nat.check(lib.rpm_singlet_gram_batch(vp(nat.ptr(V)), m, vp(nat.ptr(d_kb)), B, n // 4,
+                                                         vp(nat.ptr(P)), vp(nat.ptr(gram_ws)), gram_ws.numel(), sp))
+                    ta = np.array(tasks, dtype=_TASK)
+                    vals = torch.empty(len(ta), dtype=torch.float64, device=dev)
+                    perr = torch.empty(len(ta), dtype=torch.float64, device=dev)
+                    run_tasks(ta, True, vals, perr, None)
+                    vh = vals.cpu().numpy()
+                    ph_ = perr.cpu().numpy()
+                    for b in group:
+                        s0, s1 = spans[b]
+                        resume(b, (vh[s0:s1], ph_[s0:s1]))
+                elif phase == _ADV:
+                    kb = np.zeros(B, dtype=np.int32)
+                    rows = np.zeros(B, dtype=np.int64)
+                    for b in group:
+                        row, sel = pending[b][1]
+                        kb[b] = sel
+                        rows[b] = row
+                    d_kb = torch.from_numpy(kb).to(dev)
+                    cbuf = coeffs.index_select(0, torch.from_numpy(rows).to(dev)).contiguous()
+                    nat.check(lib.rpm_krylov_combine_batch(
+                        vp(nat.ptr(V)), m, vp(nat.ptr(d_kb)), B, n, vp(nat.ptr(cbuf)), m, vp(nat.ptr(norm)),
+                        vp(nat.ptr(states)), n, sp))
+                    nv = norms()
+                    for b in group:
+                        resume(b, abs(float(nv[b]) - 1.0))
+
+        out = []
+        for b in range(B):
+            values, bounds, errors, stats, record = results[b]
+            out.append(PropagationResult(times, values, bounds, errors, None, None, stats, record, states[b]))
+        return out

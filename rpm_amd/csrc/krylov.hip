@@ -31,6 +31,17 @@ constexpr int kMaxCand = 16;
 constexpr double kSplitZ = 1300.0;                               // largest ht per sub-step
 constexpr int kMaxTerms = static_cast<int>(1.5 * kSplitZ) + 41;  // Bessel terms per sub-step
 
+struct KrylovTask {
+  int b;       // trajectory
+  int dim;     // Krylov dimension of the evaluation (<= that trajectory's basis size)
+  int low;     // embedded lower dimension (0: none)
+  int out;     // output slot
+  double t;    // time offset
+  int kind;    // 0 candidate, 1 output value
+  int pad;
+};
+static_assert(sizeof(KrylovTask) == 32, "task layout is part of the C ABI");
+
 struct CandArgs {
   int ncand;
   int dims[kMaxCand];
@@ -119,12 +130,9 @@ __device__ c128 cheb_expm_e1(int n, const double* __restrict__ alpha, const doub
   const double hi = wave_max(lane < n ? a + fabs(bp) + fabs(b) : -1e300);
   const double c0 = 0.5 * (lo + hi);
   double h = 0.5 * (hi - lo);
-  if (!(h > 0.0) || t == 0.0) {
-    double sn, cs;
-    sincos(c0 * t, &sn, &cs);
-    return lane == 0 ? cmake(cs, -sn) : cmake(0.0, 0.0);
-  }
-  h *= 1.0 + 1e-12;
+  // Every wave of the block reaches the one barrier below, also in the trivial case.
+  const bool trivial = !(h > 0.0) || t == 0.0;
+  h = trivial ? 1.0 : h * (1.0 + 1e-12);
   const double ih = 1.0 / h;
   const double dx = lane < n ? (a - c0) * ih : 0.0;
   const double ox = b * ih, oxp = bp * ih;
@@ -132,8 +140,13 @@ __device__ c128 cheb_expm_e1(int n, const double* __restrict__ alpha, const doub
   const int nsub = ztot > kSplitZ ? static_cast<int>(ceil(ztot / kSplitZ)) : 1;
   const double tsub = t / nsub, z = h * tsub;
   const int K = static_cast<int>(1.5 * z) + 40;
-  if (lane == 0) bessel_series(z, K, bess);
+  if (!trivial && lane == 0) bessel_series(z, K, bess);
   __syncthreads();
+  if (trivial) {
+    double sn, cs;
+    sincos(c0 * t, &sn, &cs);
+    return lane == 0 ? cmake(cs, -sn) : cmake(0.0, 0.0);
+  }
   double re, im;
   cheb_real(lane == 0 ? 1.0 : 0.0, dx, ox, oxp, bess, K, re, im);
   for (int s = 1; s < nsub; ++s) {
@@ -215,6 +228,64 @@ __global__ void __launch_bounds__(128) k_outputs(const double* __restrict__ alph
   }
 }
 
+// Batched form of k_candidates / k_outputs: one block per task, tasks of
+// many trajectories in one launch.  Trajectory b's tridiagonal is
+// (alpha + b*ab_stride, beta + b*ab_stride), its norm norm[b] and its
+// projected singlet matrix P + b*pstride (dim x dim, row-major).
+//  kind 0 (candidate): v0[out] = safety*norm*||c_dim - pad(c_low)|| (0 if low >= dim),
+//                      coeffs[out*m_stride + i] = c_dim padded with zeros;
+//  kind 1 (output):    v0[out] = norm^2 Re(c^H P c), v1[out] = point error (0 if low == 0),
+//                      coeffs row out written if coeffs is not NULL.
+__global__ void __launch_bounds__(128) k_tasks(const KrylovTask* __restrict__ tasks, const double* __restrict__ alpha,
+                                               const double* __restrict__ beta, int ab_stride,
+                                               const double* __restrict__ norm, const c128* __restrict__ P,
+                                               long long pstride, double safety, double* __restrict__ v0,
+                                               double* __restrict__ v1, c128* __restrict__ coeffs, int m_stride) {
+  __shared__ double bess[2][kMaxTerms + 1];
+  __shared__ c128 ch[kMaxKrylov], cl[kMaxKrylov];
+  KrylovTask tk = tasks[blockIdx.x];
+  tk.dim = min(max(tk.dim, 1), ab_stride);   // stay inside the trajectory's tridiagonal whatever the task says
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const double* al = alpha + static_cast<long long>(tk.b) * ab_stride;
+  const double* be = beta + static_cast<long long>(tk.b) * ab_stride;
+  const bool has_low = tk.low > 0 && tk.low < tk.dim;
+  const c128 c = cheb_expm_e1(wave == 0 ? tk.dim : (has_low ? tk.low : 1), al, be, tk.t, bess[wave]);
+  if (wave == 0) ch[lane] = c;
+  else cl[lane] = c;
+  __syncthreads();
+  if (wave != 0) return;
+  const int dim = tk.dim;
+  double q = 0.0, e2 = 0.0;
+  if (lane < dim) {
+    if (has_low) {
+      c128 df = c;
+      if (lane < tk.low) df = csub(df, cl[lane]);
+      e2 = df.re * df.re + df.im * df.im;
+    }
+    if (tk.kind == 1 && P) {
+      const c128* Pb = P + static_cast<long long>(tk.b) * pstride;
+      c128 pc{0.0, 0.0};
+      for (int j = 0; j < dim; ++j) pc = cfmaconj(Pb[j * dim + lane], ch[j], pc);
+      q = c.re * pc.re + c.im * pc.im;
+    }
+  }
+  if (coeffs)
+    for (int i = lane; i < m_stride; i += kWave)
+      coeffs[static_cast<long long>(tk.out) * m_stride + i] = i < dim ? ch[i] : cmake(0.0, 0.0);
+  q = wave_sum(q);
+  e2 = wave_sum(e2);
+  if (lane == 0) {
+    const double nr = norm[tk.b];
+    const double pe = has_low ? safety * nr * sqrt(e2) : 0.0;
+    if (tk.kind == 0) {
+      v0[tk.out] = pe;
+    } else {
+      v0[tk.out] = nr * nr * q;
+      if (v1) v1[tk.out] = pe;
+    }
+  }
+}
+
 }  // namespace rpm
 
 using namespace rpm;
@@ -262,6 +333,25 @@ extern "C" int rpm_krylov_outputs(const double* d_alpha, const double* d_beta, c
   hipLaunchKernelGGL(k_outputs, dim3(nt), dim3(128), 0, static_cast<hipStream_t>(stream), d_alpha, d_beta, d_norm,
                      static_cast<const c128*>(d_P), sel, low, d_offsets, safety, d_values, d_point_err,
                      static_cast<c128*>(d_coeffs_out));
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+extern "C" int rpm_krylov_tasks(const void* d_tasks, int ntasks, const double* d_alpha, const double* d_beta,
+                                int ab_stride, const double* d_norm, const void* d_P, long long p_stride,
+                                double safety, double* d_v0, double* d_v1, void* d_coeffs, int m_stride,
+                                void* stream) {
+  if (!d_tasks || !d_alpha || !d_beta || !d_norm || !d_v0 || ntasks < 0 || ab_stride < 1)
+    return fail(RPM_E_INVALID, "rpm_krylov_tasks: bad arguments");
+  if (m_stride < 1 || m_stride > kMaxKrylov || ab_stride > kMaxKrylov)
+    return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported on the device", kMaxKrylov);
+  if (d_P && p_stride < static_cast<long long>(ab_stride) * ab_stride)
+    return fail(RPM_E_INVALID, "rpm_krylov_tasks: P stride %lld below %d^2", p_stride, ab_stride);
+  if (ntasks == 0) return RPM_OK;
+  hipLaunchKernelGGL(k_tasks, dim3(ntasks), dim3(128), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const KrylovTask*>(d_tasks), d_alpha, d_beta, ab_stride, d_norm,
+                     static_cast<const c128*>(d_P), p_stride, safety, d_v0, d_v1, static_cast<c128*>(d_coeffs),
+                     m_stride);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

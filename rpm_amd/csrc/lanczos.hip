@@ -1,13 +1,16 @@
 // Lanczos basis with classical Gram-Schmidt applied twice (CGS2), norms,
-// singlet projection of a Krylov basis and the basis-combination update.
+// singlet projection of a Krylov basis and the basis-combination update, for
+// one trajectory or a batch of trajectories (grid.y = trajectory).
 //
 // Follows R/rpm_statevector_engine.py:651-688 (basis), 492-497 (singlet
 // population), 521-529 (projected observable), 904 (endpoint update).
 //
 // All reductions are two-stage with a fixed summation order, so results are
-// bitwise reproducible run to run.  Every basis kernel reads a device flag
-// and returns immediately once the basis has broken down (beta <= tol), so
-// the host never waits inside the basis loop.
+// bitwise reproducible run to run.  Each trajectory of a batch carries a
+// BasisCtl block (active flag, basis size, target size, 1/norm): every basis
+// kernel reads it and returns at once when the trajectory is idle at the
+// current Lanczos index (breakdown, or a smaller target), so the host never
+// waits inside the basis loop.
 #include <cmath>
 #include <cstring>
 
@@ -16,25 +19,18 @@
 namespace rpm {
 
 constexpr int kThreads = 256;
-constexpr int kEPT = 4;                        // elements per thread in vector passes
+constexpr int kEPT = 4;                        // elements per thread in the norm reductions
 constexpr int kChunk = kThreads * kEPT;        // elements per block
-constexpr int kColGroup = 8;                   // basis columns per block in the dot pass
-
-struct BasisCtl {
-  int active;       // 0 once the basis broke down
-  int size;         // basis size built so far / final
-  double scale;     // 1/norm of the vector that the next matvec normalises
-  double pad;
-};
+constexpr int kMaxBasis = 64;                  // largest Krylov dimension on the device
 
 // ---------------------------------------------------------------- plain reductions
 
 // partial[b * nblk + blk] = sum |x|^2 over the block's chunk of vector b
 __global__ void __launch_bounds__(kThreads) k_sumsq(const c128* __restrict__ x, long long n, long long ld,
-                                                    double* __restrict__ partial, const int* active) {
-  if (active && *active == 0) return;
-  __shared__ double sm[kThreads / kWave];
+                                                    double* __restrict__ partial, const int* __restrict__ targets) {
   const long long b = blockIdx.y;
+  if (targets && targets[b] <= 0) return;   // vector not part of this basis call
+  __shared__ double sm[kThreads / kWave];
   const c128* v = x + b * ld;
   const long long base = static_cast<long long>(blockIdx.x) * kChunk + threadIdx.x;
   double s = 0.0;
@@ -87,116 +83,115 @@ __global__ void __launch_bounds__(kThreads) k_finish_sum(const double* __restric
 
 // ---------------------------------------------------------------- basis kernels
 
-// Norm of the starting state: norm, 1/norm into ctl, reset the control block.
-__global__ void k_basis_start(const double* __restrict__ partial, int nblk, double* __restrict__ norm_out,
-                              BasisCtl* ctl, int count) {
+// Trajectory b takes part in the step of Lanczos index j that still needs
+// `need` more vectors after j: active && j + need < target.
+__device__ inline bool basis_live(const BasisCtl* ctl, int b, int j, int need) {
+  const BasisCtl& c = ctl[b];
+  return c.active && j + need < c.target;
+}
+
+// Norm of each starting state; resets the control blocks.
+__global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ norm_out, BasisCtl* ctl,
+                                                          const int* __restrict__ targets, int uniform, long long n) {
   __shared__ double sm[kThreads / kWave];
+  const int b = blockIdx.x;
   double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kThreads) s += partial[i];
+  for (int i = threadIdx.x; i < nblk; i += kThreads) s += partial[static_cast<long long>(b) * nblk + i];
   s = block_sum(s, sm);
   if (threadIdx.x == 0) {
     const double nrm = std::sqrt(s);
-    *norm_out = nrm;
-    ctl->active = nrm > 0.0 ? 1 : 0;
-    ctl->size = nrm > 0.0 ? count : 0;
-    ctl->scale = nrm > 0.0 ? 1.0 / nrm : 0.0;
-  }
-}
-
-// partial[c * nchunk + blk] = sum_x conj(V_c[x]) w[x] over the block's chunk, for c in the group.
-__global__ void __launch_bounds__(kThreads) k_dots(const c128* __restrict__ V, long long n, int k,
-                                                   const c128* __restrict__ w, c128* __restrict__ partial,
-                                                   const BasisCtl* ctl) {
-  if (ctl && ctl->active == 0) return;
-  __shared__ c128 sm[kThreads / kWave];
-  const int nchunk = gridDim.x;
-  const long long base = static_cast<long long>(blockIdx.x) * kChunk + threadIdx.x;
-  c128 wv[kEPT];
-#pragma unroll
-  for (int e = 0; e < kEPT; ++e) {
-    const long long i = base + static_cast<long long>(e) * kThreads;
-    wv[e] = i < n ? w[i] : cmake(0.0, 0.0);
-  }
-  const int c0 = blockIdx.y * kColGroup;
-  const int c1 = min(k, c0 + kColGroup);
-  for (int c = c0; c < c1; ++c) {
-    const c128* v = V + static_cast<long long>(c) * n;
-    c128 s{0.0, 0.0};
-#pragma unroll
-    for (int e = 0; e < kEPT; ++e) {
-      const long long i = base + static_cast<long long>(e) * kThreads;
-      if (i < n) s = cfmaconj(v[i], wv[e], s);
+    int count = targets ? min(targets[b], uniform) : uniform;
+    if (count > n) count = static_cast<int>(n);
+    if (count <= 0) {
+      // not part of this call: its norm, basis and tridiagonal stay as they are
+      ctl[b].active = 0;
+      ctl[b].size = 0;
+      ctl[b].target = -1;
+      ctl[b].scale = 0.0;
+    } else {
+      norm_out[b] = nrm;
+      const bool live = nrm > 0.0;
+      ctl[b].active = live ? 1 : 0;
+      ctl[b].size = live ? count : 0;
+      ctl[b].target = live ? count : 0;
+      ctl[b].scale = live ? 1.0 / nrm : 0.0;
     }
-    s = block_sum(s, sm);
-    if (threadIdx.x == 0) partial[static_cast<long long>(c) * nchunk + blockIdx.x] = s;
   }
 }
 
-// h[c] = sum_blk partial[c][blk]; optionally alpha[alpha_slot] = Re h[alpha_col].
-__global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict__ partial, int nchunk,
-                                                          c128* __restrict__ h, double* __restrict__ alpha,
-                                                          int alpha_col, int alpha_slot, const BasisCtl* ctl) {
-  if (ctl && ctl->active == 0) return;
+// h[c] = sum_blk partial[c][blk]; optionally alpha[j] = Re h[alpha_col].
+__global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict__ partial, int nblk, long long pstride,
+                                                          c128* __restrict__ h, int hstride, double* __restrict__ alpha,
+                                                          int astride, int alpha_col, int j, int need,
+                                                          const BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  if (!basis_live(ctl, b, j, need)) return;
   __shared__ c128 sm[kThreads / kWave];
   const int c = blockIdx.x;
+  const c128* p = partial + b * pstride + static_cast<long long>(c) * nblk;
   c128 s{0.0, 0.0};
-  for (int i = threadIdx.x; i < nchunk; i += kThreads) s = cadd(s, partial[static_cast<long long>(c) * nchunk + i]);
+  for (int i = threadIdx.x; i < nblk; i += kThreads) s = cadd(s, p[i]);
   s = block_sum(s, sm);
   if (threadIdx.x == 0) {
-    h[c] = s;
-    if (alpha && c == alpha_col) alpha[alpha_slot] = s.re;
+    h[b * hstride + c] = s;
+    if (alpha && c == alpha_col) alpha[b * astride + j] = s.re;
   }
 }
 
-// w -= sum_c V_c h_c over the chunk (one element per thread); optional partial sums of |w|^2.
+// w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2.
 constexpr int kUpdChunk = kThreads;
-__global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V, long long n, int k,
-                                                     const c128* __restrict__ h, c128* __restrict__ w,
-                                                     double* __restrict__ norm_partial, const BasisCtl* ctl) {
-  if (ctl && ctl->active == 0) return;
+__global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V, long long vstride, long long n, int k,
+                                                     const c128* __restrict__ h, int hstride, c128* __restrict__ W,
+                                                     double* __restrict__ norm_partial, int j, const BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  if (!basis_live(ctl, b, j, 1)) return;
   __shared__ double sm[kThreads / kWave];
+  const c128* Vb = V + b * vstride;
+  const c128* hb = h + b * hstride;
+  c128* w = W + b * n;
   const long long i = static_cast<long long>(blockIdx.x) * kUpdChunk + threadIdx.x;
   c128 wv = i < n ? w[i] : cmake(0.0, 0.0);
   if (i < n) {
     int c = 0;
     for (; c + 4 <= k; c += 4) {
-      const c128 v0 = V[static_cast<long long>(c) * n + i], v1 = V[static_cast<long long>(c + 1) * n + i];
-      const c128 v2 = V[static_cast<long long>(c + 2) * n + i], v3 = V[static_cast<long long>(c + 3) * n + i];
-      wv = cfms(v0, h[c], wv);
-      wv = cfms(v1, h[c + 1], wv);
-      wv = cfms(v2, h[c + 2], wv);
-      wv = cfms(v3, h[c + 3], wv);
+      const c128 v0 = Vb[static_cast<long long>(c) * n + i], v1 = Vb[static_cast<long long>(c + 1) * n + i];
+      const c128 v2 = Vb[static_cast<long long>(c + 2) * n + i], v3 = Vb[static_cast<long long>(c + 3) * n + i];
+      wv = cfms(v0, hb[c], wv);
+      wv = cfms(v1, hb[c + 1], wv);
+      wv = cfms(v2, hb[c + 2], wv);
+      wv = cfms(v3, hb[c + 3], wv);
     }
-    for (; c < k; ++c) wv = cfms(V[static_cast<long long>(c) * n + i], h[c], wv);
+    for (; c < k; ++c) wv = cfms(Vb[static_cast<long long>(c) * n + i], hb[c], wv);
     w[i] = wv;
   }
-  if (norm_partial) {
-    double s = fma(wv.re, wv.re, wv.im * wv.im);
-    s = block_sum(s, sm);
-    if (threadIdx.x == 0) norm_partial[blockIdx.x] = s;
-  }
+  double s = fma(wv.re, wv.re, wv.im * wv.im);
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) norm_partial[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
 }
 
 // Fused CGS pass over an LDS-staged chunk of the basis: optionally
 // w <- w - V h (update), then partial[c][blk] = sum_x conj(V_c[x]) w[x] for
 // all c < k.  Each block walks kGsPer chunks of kGsChunk elements; V is read
-// from HBM once per pass.  k <= kGsMaxK.
+// from HBM once per pass.  k <= kMaxBasis.
 constexpr int kGsChunk = 32;
 constexpr int kGsPer = 4;
-constexpr int kGsMaxK = 64;
-__global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, long long n, int k,
-                                                 const c128* __restrict__ h, c128* __restrict__ w,
-                                                 c128* __restrict__ partial, const BasisCtl* ctl) {
-  if (ctl && ctl->active == 0) return;
-  __shared__ c128 vs[kGsMaxK][kGsChunk + 1];
+__global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, long long vstride, long long n, int k,
+                                                 const c128* __restrict__ h, int hstride, c128* __restrict__ W,
+                                                 c128* __restrict__ partial, long long pstride, int j, int need,
+                                                 const BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  if (!basis_live(ctl, b, j, need)) return;
+  __shared__ c128 vs[kMaxBasis][kGsChunk + 1];
   __shared__ c128 ws[kGsChunk];
   __shared__ c128 red[256];
-  __shared__ c128 hs[kGsMaxK];
+  __shared__ c128 hs[kMaxBasis];
+  const c128* Vb = V + b * vstride;
+  c128* w = W + b * n;
   const int tid = threadIdx.x;
   const int nblk = gridDim.x;
-  if (h && tid < k) hs[tid] = h[tid];
-  // dot mapping: column c = tid % 64, part q = tid / 64 (4 parts of 8 elements)
-  const int dc = tid % 64, dq = tid / 64;
+  if (h && tid < k) hs[tid] = h[b * hstride + tid];
+  const int dc = tid % 64, dq = tid / 64;   // dot mapping: column dc, 8-element part dq
   c128 dot{0.0, 0.0};
   for (int it = 0; it < kGsPer; ++it) {
     const long long x0 = (static_cast<long long>(blockIdx.x) * kGsPer + it) * kGsChunk;
@@ -205,7 +200,7 @@ __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, lon
     for (int s = tid; s < k * kGsChunk; s += 256) {
       const int c = s / kGsChunk, i = s % kGsChunk;
       const long long x = x0 + i;
-      vs[c][i] = x < n ? V[static_cast<long long>(c) * n + x] : cmake(0.0, 0.0);
+      vs[c][i] = x < n ? Vb[static_cast<long long>(c) * n + x] : cmake(0.0, 0.0);
     }
     if (tid < kGsChunk) {
       const long long x = x0 + tid;
@@ -213,15 +208,14 @@ __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, lon
     }
     __syncthreads();
     if (h) {
-      // 8 threads per element, each over k/8 columns
-      const int i = tid / 8, part = tid % 8;
+      const int i = tid / 8, part = tid % 8;   // 8 threads per element, each over k/8 columns
       c128 s{0.0, 0.0};
       for (int c = part; c < k; c += 8) s = cfma(vs[c][i], hs[c], s);
       red[tid] = s;
       __syncthreads();
       if (part == 0) {
         c128 t = red[tid];
-        for (int p = 1; p < 8; ++p) t = cadd(t, red[tid + p]);
+        for (int q = 1; q < 8; ++q) t = cadd(t, red[tid + q]);
         const c128 nw = csub(ws[i], t);
         ws[i] = nw;
         const long long x = x0 + i;
@@ -239,64 +233,76 @@ __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, lon
   __syncthreads();
   if (tid < 64 && tid < k) {
     const c128 t = cadd(cadd(red[tid], red[tid + 64]), cadd(red[tid + 128], red[tid + 192]));
-    partial[static_cast<long long>(tid) * nblk + blockIdx.x] = t;
+    partial[b * pstride + static_cast<long long>(tid) * nblk + blockIdx.x] = t;
   }
 }
 
 // beta_j = ||w||; breakdown test; scale for the next matvec.
-__global__ void __launch_bounds__(kThreads) k_finish_beta(const double* __restrict__ partial, int nchunk,
-                                                          double* __restrict__ beta, int j, double tol,
+__global__ void __launch_bounds__(kThreads) k_finish_beta(const double* __restrict__ partial, int nblk,
+                                                          double* __restrict__ beta, int bstride, int j, double tol,
                                                           BasisCtl* ctl) {
-  if (ctl->active == 0) return;
+  const int b = blockIdx.x;
+  if (!basis_live(ctl, b, j, 1)) return;
   __shared__ double sm[kThreads / kWave];
   double s = 0.0;
-  for (int i = threadIdx.x; i < nchunk; i += kThreads) s += partial[i];
+  for (int i = threadIdx.x; i < nblk; i += kThreads) s += partial[static_cast<long long>(b) * nblk + i];
   s = block_sum(s, sm);
   if (threadIdx.x == 0) {
-    const double b = std::sqrt(s);
-    beta[j] = b;
-    if (b <= tol) {
-      ctl->active = 0;
-      ctl->size = j + 1;
+    const double bj = std::sqrt(s);
+    beta[b * bstride + j] = bj;
+    if (bj <= tol) {
+      ctl[b].active = 0;
+      ctl[b].size = j + 1;
     } else {
-      ctl->scale = 1.0 / b;
+      ctl[b].scale = 1.0 / bj;
     }
   }
 }
 
-__global__ void k_basis_info(const BasisCtl* ctl, int* info) {
-  if (threadIdx.x == 0) {
-    info[0] = ctl->size;
-    info[1] = ctl->active;
+__global__ void k_basis_info(const BasisCtl* ctl, int* info, int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < batch && ctl[b].target >= 0) {
+    info[b * 4 + 0] = ctl[b].size;
+    info[b * 4 + 1] = ctl[b].active;
   }
 }
 
 // ---------------------------------------------------------------- combine / gram
 
-// out[x] = scale * sum_j V_j[x] c_j
-__global__ void __launch_bounds__(kThreads) k_combine(const c128* __restrict__ V, int k, long long n,
-                                                      const c128* __restrict__ c, const double* __restrict__ scale,
-                                                      c128* __restrict__ out) {
+// out_b[x] = scale_b * sum_{j<k_b} V_b[j][x] c_b[j]   (k_b from the array or the scalar k)
+__global__ void __launch_bounds__(kThreads) k_combine(const c128* __restrict__ V, long long vstride, int k,
+                                                      const int* __restrict__ kb, long long n,
+                                                      const c128* __restrict__ c, int cstride,
+                                                      const double* __restrict__ scale, c128* __restrict__ out,
+                                                      long long ostride) {
+  const int b = blockIdx.y;
+  const int kk = kb ? kb[b] : k;
+  if (kk <= 0) return;
   const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
+  const c128* Vb = V + b * vstride;
+  const c128* cb = c + b * cstride;
   c128 s{0.0, 0.0};
-  for (int j = 0; j < k; ++j) s = cfma(V[static_cast<long long>(j) * n + i], c[j], s);
-  const double sc = scale ? *scale : 1.0;
-  out[i] = cscale(s, sc);
+  for (int j = 0; j < kk; ++j) s = cfma(Vb[static_cast<long long>(j) * n + i], cb[j], s);
+  const double sc = scale ? scale[b] : 1.0;
+  out[b * ostride + i] = cscale(s, sc);
 }
 
 constexpr int kGramTile = 64;
 constexpr int kGramChunk = 16;
 
-// Partial S[a][b] = sum_n conj(A_a[n]) A_b[n] for a in [a0, a0+64), b in [b0, b0+64),
-// A_a[n] = (V_a[nn + n] - V_a[2 nn + n]) / sqrt2.  One 4x4 register block per thread.
-__global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, int k, long long nn, int a0, int b0,
-                                              long long n_per_block, c128* __restrict__ partial) {
+// Partial S[a][b'] = sum_n conj(A_a[n]) A_b'[n] for a, b' < k <= 64, A_a[n] = (V_a[nn + n] - V_a[2 nn + n]) / sqrt2.
+// One 4x4 register block per thread; grid (nblk, batch).
+__global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long long vstride, const int* __restrict__ kb,
+                                              int k1, long long nn, long long n_per_block, c128* __restrict__ partial) {
+  const int bt = blockIdx.y;
+  const int k = kb ? kb[bt] : k1;
+  if (k <= 0) return;
   __shared__ c128 ra[kGramChunk][kGramTile + 1];
-  __shared__ c128 rb[kGramChunk][kGramTile + 1];
   const int tid = threadIdx.x;
   const int ia = tid / 16, ib = tid % 16;
   const long long N = 4 * nn;
+  const c128* Vb = V + bt * vstride;
   const long long n0 = static_cast<long long>(blockIdx.x) * n_per_block;
   const long long n1 = min(nn, n0 + n_per_block);
   const double r2 = 1.0 / std::sqrt(2.0);
@@ -306,25 +312,16 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, int k,
 #pragma unroll
     for (int v = 0; v < 4; ++v) acc[u][v] = cmake(0.0, 0.0);
   for (long long c0 = n0; c0 < n1; c0 += kGramChunk) {
-    // stage: 64 rows x kGramChunk columns for each side
     for (int s = tid; s < kGramTile * kGramChunk; s += 256) {
       const int row = s / kGramChunk, col = s % kGramChunk;
-      const long long n = c0 + col;
-      c128 va{0.0, 0.0}, vb{0.0, 0.0};
-      if (n < n1) {
-        if (a0 + row < k) {
-          const c128* v = V + static_cast<long long>(a0 + row) * N;
-          const c128 p = v[nn + n], q = v[2 * nn + n];
-          va = cmake((p.re - q.re) * r2, (p.im - q.im) * r2);
-        }
-        if (b0 + row < k) {
-          const c128* v = V + static_cast<long long>(b0 + row) * N;
-          const c128 p = v[nn + n], q = v[2 * nn + n];
-          vb = cmake((p.re - q.re) * r2, (p.im - q.im) * r2);
-        }
+      const long long nidx = c0 + col;
+      c128 va{0.0, 0.0};
+      if (nidx < n1 && row < k) {
+        const c128* v = Vb + static_cast<long long>(row) * N;
+        const c128 p = v[nn + nidx], q = v[2 * nn + nidx];
+        va = cmake((p.re - q.re) * r2, (p.im - q.im) * r2);
       }
       ra[col][row] = va;
-      rb[col][row] = vb;
     }
     __syncthreads();
     for (int col = 0; col < kGramChunk; ++col) {
@@ -332,7 +329,7 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, int k,
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         xa[u] = ra[col][ia * 4 + u];
-        xb[u] = rb[col][ib * 4 + u];
+        xb[u] = ra[col][ib * 4 + u];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u)
@@ -341,32 +338,29 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, int k,
     }
     __syncthreads();
   }
-  c128* out = partial + static_cast<long long>(blockIdx.x) * kGramTile * kGramTile;
+  c128* out = partial + (static_cast<long long>(bt) * gridDim.x + blockIdx.x) * kGramTile * kGramTile;
 #pragma unroll
   for (int u = 0; u < 4; ++u)
 #pragma unroll
     for (int v = 0; v < 4; ++v) out[(ia * 4 + u) * kGramTile + ib * 4 + v] = acc[u][v];
 }
 
-// S tile = sum over blocks of the partial tiles, written into S (k x k row-major)
-__global__ void __launch_bounds__(256) k_gram_reduce(const c128* __restrict__ partial, int nblk, int k, int a0,
-                                                     int b0, c128* __restrict__ S) {
-  const int e = blockIdx.x * 256 + threadIdx.x;
-  if (e >= kGramTile * kGramTile) return;
-  const int a = a0 + e / kGramTile, b = b0 + e % kGramTile;
-  if (a >= k || b >= k) return;
-  c128 s{0.0, 0.0};
-  for (int i = 0; i < nblk; ++i) s = cadd(s, partial[static_cast<long long>(i) * kGramTile * kGramTile + e]);
-  S[a * k + b] = s;
-}
-
-// P = (S + S^H) / 2
-__global__ void k_hermitize(const c128* __restrict__ S, int k, c128* __restrict__ P) {
+// P_b = Hermitian part of sum over blocks of the partial tiles (k x k row-major at P + b*pstride).
+__global__ void __launch_bounds__(256) k_gram_reduce(const c128* __restrict__ partial, int nblk,
+                                                     const int* __restrict__ kb, int k1, c128* __restrict__ P,
+                                                     long long pstride) {
+  const int bt = blockIdx.y;
+  const int k = kb ? kb[bt] : k1;
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= k * k) return;
-  const int a = e / k, b = e % k;
-  const c128 x = S[a * k + b], y = S[b * k + a];
-  P[e] = cmake(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+  const int a = e / k, c = e % k;
+  const c128* pb = partial + static_cast<long long>(bt) * nblk * kGramTile * kGramTile;
+  c128 x{0.0, 0.0}, y{0.0, 0.0};
+  for (int i = 0; i < nblk; ++i) {
+    x = cadd(x, pb[static_cast<long long>(i) * kGramTile * kGramTile + a * kGramTile + c]);
+    y = cadd(y, pb[static_cast<long long>(i) * kGramTile * kGramTile + c * kGramTile + a]);
+  }
+  P[bt * pstride + e] = cmake(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
 }
 
 }  // namespace rpm
@@ -389,8 +383,8 @@ extern "C" int rpm_norm(const void* d_x, long long n, int batch, long long ld, d
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int nblk = ceil_div(n, kChunk);
   double* part = static_cast<double*>(d_ws);
-  hipLaunchKernelGGL(k_sumsq, dim3(nblk, batch), dim3(kThreads), 0, s, static_cast<const c128*>(d_x), n, ld,
-                     part, nullptr);
+  hipLaunchKernelGGL(k_sumsq, dim3(nblk, batch), dim3(kThreads), 0, s, static_cast<const c128*>(d_x), n, ld, part,
+                     nullptr);
   RPM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_finish_sum, dim3(batch), dim3(kThreads), 0, s, part, nblk, d_out, 1);
   RPM_LAUNCH_CHECK();
@@ -406,8 +400,8 @@ extern "C" int rpm_singlet_population(const void* d_x, long long nn, int batch, 
   hipStream_t s = static_cast<hipStream_t>(stream);
   const int nblk = ceil_div(nn, kChunk);
   double* part = static_cast<double*>(d_ws);
-  hipLaunchKernelGGL(k_singlet_sumsq, dim3(nblk, batch), dim3(kThreads), 0, s, static_cast<const c128*>(d_x), nn,
-                     ld, part);
+  hipLaunchKernelGGL(k_singlet_sumsq, dim3(nblk, batch), dim3(kThreads), 0, s, static_cast<const c128*>(d_x), nn, ld,
+                     part);
   RPM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_finish_sum, dim3(batch), dim3(kThreads), 0, s, part, nblk, d_out, 0);
   RPM_LAUNCH_CHECK();
@@ -416,129 +410,175 @@ extern "C" int rpm_singlet_population(const void* d_x, long long nn, int batch, 
 
 namespace {
 struct BasisWs {
-  c128* w[2];
-  c128* partial;     // [m][nchunk]
-  c128* h;           // [m]
-  double* npart;     // [nchunk]
-  BasisCtl* ctl;
+  c128* w[2];        // [batch][n] each
+  c128* partial;     // [batch][m][nblk]
+  long long pstride;
+  int nblk;
+  c128* h;           // [batch][m]
+  double* npart;     // [batch][nupd]
+  double* sq;        // [batch][nchunk]
+  BasisCtl* ctl;     // [batch]
   size_t bytes;
 };
 
-BasisWs basis_layout(char* base, long long n, int m) {
-  long long nchunk = (n + kChunk - 1) / kChunk;
-  const long long gsb = (n + kGsChunk * kGsPer - 1) / (kGsChunk * kGsPer);
-  const long long nupd = (n + kUpdChunk - 1) / kUpdChunk;
-  if (gsb > nchunk) nchunk = gsb;
-  if (nupd > nchunk) nchunk = nupd;
+BasisWs basis_layout(char* base, long long n, int m, int batch) {
+  const int nchunk = ceil_div(n, kChunk);
+  const int gsb = ceil_div(n, kGsChunk * kGsPer);
+  const int nupd = ceil_div(n, kUpdChunk);
   size_t off = 0;
-  auto take = [&](size_t b) { size_t o = off; off += (b + 255) & ~size_t(255); return o; };
-  const size_t o_w0 = take(n * 16), o_w1 = take(n * 16), o_p = take(static_cast<size_t>(m) * nchunk * 16);
-  const size_t o_h = take(static_cast<size_t>(m) * 16), o_np = take(nchunk * 8), o_ctl = take(sizeof(BasisCtl));
+  auto take = [&](size_t b) {
+    size_t o = off;
+    off += (b + 255) & ~size_t(255);
+    return o;
+  };
+  const size_t B = static_cast<size_t>(batch);
+  const size_t o_w0 = take(B * n * 16), o_w1 = take(B * n * 16);
+  const size_t o_p = take(B * m * gsb * 16), o_h = take(B * m * 16), o_np = take(B * nupd * 8);
+  const size_t o_sq = take(B * nchunk * 8), o_ctl = take(B * sizeof(BasisCtl));
   BasisWs w;
   w.w[0] = reinterpret_cast<c128*>(base + o_w0);
   w.w[1] = reinterpret_cast<c128*>(base + o_w1);
   w.partial = reinterpret_cast<c128*>(base + o_p);
+  w.pstride = static_cast<long long>(m) * gsb;
+  w.nblk = gsb;
   w.h = reinterpret_cast<c128*>(base + o_h);
   w.npart = reinterpret_cast<double*>(base + o_np);
+  w.sq = reinterpret_cast<double*>(base + o_sq);
   w.ctl = reinterpret_cast<BasisCtl*>(base + o_ctl);
   w.bytes = off;
   return w;
+}
+
+// Shared implementation of the single and batched basis builders.
+int basis_batch(const rpm_operator* op, int batch, const c128* states, long long sld, const int* d_targets,
+                int max_target, const c128* pairs, c128* V, long long vstride, int m, double* alpha, double* beta,
+                int* info, double* norm, double tol, void* ws_ptr, size_t ws_bytes, hipStream_t s) {
+  const long long n = op->n;
+  if (max_target > m || max_target < 1) return fail(RPM_E_INVALID, "target %d outside 1..%d", max_target, m);
+  if (m > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
+  BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch);
+  if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "Lanczos workspace %zu < %zu", ws_bytes, w.bytes);
+  const int nchunk = ceil_div(n, kChunk);
+  const int nupd = ceil_div(n, kUpdChunk);
+  const int count = static_cast<int>(max_target < n ? max_target : n);
+  hipLaunchKernelGGL(k_sumsq, dim3(nchunk, batch), dim3(kThreads), 0, s, states, n, sld, w.sq, d_targets);
+  RPM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, d_targets, count, n);
+  RPM_LAUNCH_CHECK();
+  for (int j = 0; j < count; ++j) {
+    const c128* in = j == 0 ? states : w.w[j % 2];
+    c128* out = w.w[(j + 1) % 2];
+    ApplyCtl A;
+    A.ctl = w.ctl;
+    A.j = j;
+    A.copy = V + static_cast<long long>(j) * n;
+    A.copy_ld = vstride;
+    A.pairs = pairs;
+    int rc = operator_apply_ex(op, in, out, batch, j == 0 ? sld : n, A, s);
+    if (rc) return rc;
+    if (j == count - 1) {
+      // last vector of the longest basis: only alpha_j = Re <v_j, H v_j> is needed
+      hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V + static_cast<long long>(j) * n, vstride,
+                         n, 1, nullptr, m, out, w.partial, w.pstride, j, 0, w.ctl);
+      RPM_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_reduce_cols, dim3(1, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m,
+                         alpha, m, 0, j, 0, w.ctl);
+      RPM_LAUNCH_CHECK();
+      break;
+    }
+    const int k = j + 1;
+    // pass 1: h1 = V^H w, alpha_j = Re h1[j] (a basis whose target is j + 1 stops here)
+    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, nullptr, m, out, w.partial,
+                       w.pstride, j, 0, w.ctl);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m, alpha,
+                       m, j, j, 0, w.ctl);
+    RPM_LAUNCH_CHECK();
+    // pass 2: w -= V h1 and h2 = V^H w from the same staged chunk
+    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, w.h, m, out, w.partial,
+                       w.pstride, j, 1, w.ctl);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m,
+                       nullptr, m, 0, j, 1, w.ctl);
+    RPM_LAUNCH_CHECK();
+    // pass 3: w -= V h2, ||w||
+    hipLaunchKernelGGL(k_update, dim3(nupd, batch), dim3(kThreads), 0, s, V, vstride, n, k, w.h, m, out, w.npart, j,
+                       w.ctl);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_finish_beta, dim3(batch), dim3(kThreads), 0, s, w.npart, nupd, beta, m, j, tol, w.ctl);
+    RPM_LAUNCH_CHECK();
+  }
+  hipLaunchKernelGGL(k_basis_info, dim3(ceil_div(batch, 64)), dim3(64), 0, s, w.ctl, info, batch);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
 }
 }  // namespace
 
 extern "C" size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m) {
   if (!op || m < 1) return 0;
-  return basis_layout(nullptr, op->n, m).bytes;
+  return basis_layout(nullptr, op->n, m, 1).bytes;
+}
+
+extern "C" size_t rpm_lanczos_batch_workspace_bytes(const rpm_operator* op, int m, int batch) {
+  if (!op || m < 1 || batch < 1) return 0;
+  return basis_layout(nullptr, op->n, m, batch).bytes;
 }
 
 extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, void* d_basis,
-                                 double* d_alpha, double* d_beta, int* d_info, double* d_norm,
-                                 double tol, void* d_ws, size_t ws_bytes, void* stream) {
+                                 double* d_alpha, double* d_beta, int* d_info, double* d_norm, double tol, void* d_ws,
+                                 size_t ws_bytes, void* stream) {
   if (!op || !d_state || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || !d_ws || target < 1)
     return fail(RPM_E_INVALID, "rpm_lanczos_basis: bad arguments");
   const long long n = op->n;
   const int count = static_cast<int>(target < n ? target : n);
-  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count);
-  if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  const int nchunk = ceil_div(n, kChunk);
-  const int nchunk_upd = ceil_div(n, kUpdChunk);
-  const int gs_blocks = ceil_div(n, kGsChunk * kGsPer);
-  c128* V = static_cast<c128*>(d_basis);
-  const c128* x0 = static_cast<const c128*>(d_state);
+  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1);
+  if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
+  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
+                     static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
+                     d_norm, tol, d_ws, ws_bytes, s);
+}
 
-  hipLaunchKernelGGL(k_sumsq, dim3(nchunk, 1), dim3(kThreads), 0, s, x0, n, n, w.npart, nullptr);
-  RPM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_basis_start, dim3(1), dim3(kThreads), 0, s, w.npart, nchunk, d_norm, w.ctl, count);
-  RPM_LAUNCH_CHECK();
-  const int* active = &w.ctl->active;
-  for (int j = 0; j < count; ++j) {
-    const c128* in = j == 0 ? x0 : w.w[j % 2];
-    c128* out = w.w[(j + 1) % 2];
-    c128* vj = V + static_cast<long long>(j) * n;
-    int rc = operator_apply_ex(op, in, out, 1, n, &w.ctl->scale, vj, active, s);
-    if (rc) return rc;
-    if (j == count - 1) {
-      // last vector: only alpha_j = Re <v_j, H v_j> is used
-      hipLaunchKernelGGL(k_dots, dim3(nchunk, 1), dim3(kThreads), 0, s, vj, n, 1, out, w.partial, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(1), dim3(kThreads), 0, s, w.partial, nchunk, w.h, d_alpha, 0, j,
-                         w.ctl);
-      RPM_LAUNCH_CHECK();
-      break;
-    }
-    const int k = j + 1;
-    if (k <= kGsMaxK) {
-      // three passes over V: h1 = V^H w | w -= V h1, h2 = V^H w | w -= V h2, ||w||
-      hipLaunchKernelGGL(k_gs_pass, dim3(gs_blocks), dim3(256), 0, s, V, n, k, nullptr, out, w.partial, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, gs_blocks, w.h, d_alpha, j, j,
-                         w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gs_pass, dim3(gs_blocks), dim3(256), 0, s, V, n, k, w.h, out, w.partial, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, gs_blocks, w.h, nullptr, 0, 0,
-                         w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, w.npart, w.ctl);
-      RPM_LAUNCH_CHECK();
-    } else {
-      const dim3 gd(nchunk, ceil_div(k, kColGroup));
-      // pass 1: h1 = V^H w (alpha_j = Re h1[j]); w -= V h1
-      hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, d_alpha, j, j, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, nullptr, w.ctl);
-      RPM_LAUNCH_CHECK();
-      // pass 2: h2 = V^H w; w -= V h2; beta = ||w||
-      hipLaunchKernelGGL(k_dots, gd, dim3(kThreads), 0, s, V, n, k, out, w.partial, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(k), dim3(kThreads), 0, s, w.partial, nchunk, w.h, nullptr, 0, 0, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_update, dim3(nchunk_upd), dim3(kThreads), 0, s, V, n, k, w.h, out, w.npart, w.ctl);
-      RPM_LAUNCH_CHECK();
-    }
-    hipLaunchKernelGGL(k_finish_beta, dim3(1), dim3(kThreads), 0, s, w.npart, nchunk_upd, d_beta, j, tol, w.ctl);
-    RPM_LAUNCH_CHECK();
-  }
-  hipLaunchKernelGGL(k_basis_info, dim3(1), dim3(64), 0, s, w.ctl, d_info);
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
+extern "C" int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_states, long long ld,
+                                       const int* d_targets, int max_target, const void* d_pairs, void* d_basis,
+                                       int m, double* d_alpha, double* d_beta, int* d_info, double* d_norm, double tol,
+                                       void* d_ws, size_t ws_bytes, void* stream) {
+  if (!op || !d_states || !d_targets || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || !d_ws || batch < 1 ||
+      batch > 65535 || m < 1)
+    return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch: bad arguments");
+  // the first matvec reads the states and writes the packed work vectors with one stride
+  if (ld != op->n) return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch: states must be packed (ld == N)");
+  return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target,
+                     static_cast<const c128*>(d_pairs), static_cast<c128*>(d_basis),
+                     static_cast<long long>(m) * op->n, m, d_alpha, d_beta, d_info, d_norm, tol, d_ws, ws_bytes,
+                     static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rpm_krylov_combine(const void* d_basis, int k, long long n, const void* d_c, const double* d_scale,
                                   void* d_out, void* stream) {
   if (!d_basis || !d_c || !d_out || k < 1 || n < 1) return fail(RPM_E_INVALID, "rpm_krylov_combine: bad arguments");
-  hipLaunchKernelGGL(k_combine, dim3(ceil_div(n, kThreads)), dim3(kThreads), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const c128*>(d_basis), k, n, static_cast<const c128*>(d_c), d_scale,
-                     static_cast<c128*>(d_out));
+  hipLaunchKernelGGL(k_combine, dim3(ceil_div(n, kThreads), 1), dim3(kThreads), 0, static_cast<hipStream_t>(stream),
+                     static_cast<const c128*>(d_basis), 0LL, k, nullptr, n, static_cast<const c128*>(d_c), 0, d_scale,
+                     static_cast<c128*>(d_out), 0LL);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+extern "C" int rpm_krylov_combine_batch(const void* d_basis, int m, const int* d_k, int batch, long long n,
+                                        const void* d_c, int c_stride, const double* d_scale, void* d_out,
+                                        long long out_ld, void* stream) {
+  if (!d_basis || !d_k || !d_c || !d_out || batch < 1 || batch > 65535 || n < 1 || m < 1 || out_ld < n)
+    return fail(RPM_E_INVALID, "rpm_krylov_combine_batch: bad arguments");
+  hipLaunchKernelGGL(k_combine, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0,
+                     static_cast<hipStream_t>(stream), static_cast<const c128*>(d_basis),
+                     static_cast<long long>(m) * n, 0, d_k, n, static_cast<const c128*>(d_c), c_stride, d_scale,
+                     static_cast<c128*>(d_out), out_ld);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }
 
 namespace {
-long long gram_blocks(long long nn) {
+long long gram_per_block(long long nn) {
   // about 256 blocks, each an integer number of kGramChunk-wide chunks
   long long per = (nn + 255) / 256;
   per = ((per + kGramChunk - 1) / kGramChunk) * kGramChunk;
@@ -549,32 +589,47 @@ long long gram_blocks(long long nn) {
 
 extern "C" size_t rpm_singlet_gram_workspace_bytes(int k, long long nn) {
   if (k < 1 || nn < 1) return 0;
-  const long long per = gram_blocks(nn);
+  const long long per = gram_per_block(nn);
   const long long nblk = (nn + per - 1) / per;
-  return static_cast<size_t>(nblk) * kGramTile * kGramTile * 16 + static_cast<size_t>(k) * k * 16 + 512;
+  return static_cast<size_t>(nblk) * kGramTile * kGramTile * 16 + 512;
 }
+
+extern "C" size_t rpm_singlet_gram_batch_workspace_bytes(long long nn, int batch) {
+  if (nn < 1 || batch < 1) return 0;
+  return rpm_singlet_gram_workspace_bytes(1, nn) * static_cast<size_t>(batch) + 512;
+}
+
+namespace {
+int gram_batch(const c128* V, long long vstride, const int* d_k, int k1, int batch, long long nn, c128* P,
+               long long pstride, void* ws, size_t ws_bytes, hipStream_t s) {
+  const long long per = gram_per_block(nn);
+  const int nblk = static_cast<int>((nn + per - 1) / per);
+  if (ws_bytes < static_cast<size_t>(nblk) * kGramTile * kGramTile * 16 * batch)
+    return fail(RPM_E_INVALID, "singlet gram workspace too small");
+  c128* partial = static_cast<c128*>(ws);
+  hipLaunchKernelGGL(k_gram, dim3(nblk, batch), dim3(256), 0, s, V, vstride, d_k, k1, nn, per, partial);
+  RPM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gram_reduce, dim3(kGramTile * kGramTile / 256, batch), dim3(256), 0, s, partial, nblk, d_k, k1,
+                     P, pstride);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+}  // namespace
 
 extern "C" int rpm_singlet_gram(const void* d_basis, int k, long long nn, void* d_P, void* d_ws, size_t ws_bytes,
                                 void* stream) {
   if (!d_basis || !d_P || !d_ws || k < 1 || nn < 1) return fail(RPM_E_INVALID, "rpm_singlet_gram: bad arguments");
+  if (k > kGramTile) return fail(RPM_E_UNSUPPORTED, "singlet projection above dimension %d", kGramTile);
   if (ws_bytes < rpm_singlet_gram_workspace_bytes(k, nn)) return fail(RPM_E_INVALID, "rpm_singlet_gram: workspace");
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  const long long per = gram_blocks(nn);
-  const int nblk = static_cast<int>((nn + per - 1) / per);
-  c128* partial = static_cast<c128*>(d_ws);
-  c128* S = reinterpret_cast<c128*>(static_cast<char*>(d_ws) +
-                                    ((static_cast<size_t>(nblk) * kGramTile * kGramTile * 16 + 255) & ~size_t(255)));
-  for (int a0 = 0; a0 < k; a0 += kGramTile)
-    for (int b0 = 0; b0 < k; b0 += kGramTile) {
-      hipLaunchKernelGGL(k_gram, dim3(nblk), dim3(256), 0, s, static_cast<const c128*>(d_basis), k, nn, a0, b0, per,
-                         partial);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_gram_reduce, dim3(kGramTile * kGramTile / 256), dim3(256), 0, s, partial, nblk, k, a0, b0,
-                         S);
-      RPM_LAUNCH_CHECK();
-    }
-  hipLaunchKernelGGL(k_hermitize, dim3(ceil_div(static_cast<long long>(k) * k, 256)), dim3(256), 0, s, S, k,
-                     static_cast<c128*>(d_P));
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
+  return gram_batch(static_cast<const c128*>(d_basis), 0, nullptr, k, 1, nn, static_cast<c128*>(d_P), 0, d_ws, ws_bytes,
+                    static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k, int batch, long long nn, void* d_P,
+                                      void* d_ws, size_t ws_bytes, void* stream) {
+  if (!d_basis || !d_k || !d_P || !d_ws || batch < 1 || batch > 65535 || nn < 1 || m < 1 || m > kGramTile)
+    return fail(RPM_E_INVALID, "rpm_singlet_gram_batch: bad arguments");
+  return gram_batch(static_cast<const c128*>(d_basis), static_cast<long long>(m) * 4 * nn, d_k, 0, batch, nn,
+                    static_cast<c128*>(d_P), static_cast<long long>(m) * m, d_ws, ws_bytes,
+                    static_cast<hipStream_t>(stream));
 }

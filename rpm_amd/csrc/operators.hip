@@ -26,33 +26,36 @@ struct PairTables {
   long long NLR;
   const c128* dA;       // [2][NL]  diagonal from left nuclei
   const c128* dB;       // [2][NR]
-  const c128* cA;       // [4][NL]  eA-flip coefficient, index (eA*2+eB)*NL + l
-  const c128* cB;       // [4][NR]  eB-flip coefficient, index (eA*2+eB)*NR + r
+  const c128* fA;       // [2][NL]  eA flip from the left nuclei, index eA*NL + l
+  const c128* fB;       // [2][NR]
+  const c128* pair;     // [16]     the operator's own 4x4 electron matrix
   const int* offL;      // [NL][KL] flat offset of the k-th left neighbour ((l2-l)*NR)
   const int* offR;      // [NR][KR] r2 - r
   const c128* coefL;    // [NL][KL][2][2]  (eA, e')
   const c128* coefR;    // [NR][KR][2][2]  (eB, e')
-  c128 pdiag[4];        // electron-pair diagonal
-  c128 pboth[4];        // (eA,eB) -> (1-eA,1-eB)
 };
 
 struct PairOp {
   PairTables t;
   TileTables tile;      // valid when use_tile
   bool use_tile;
-  int kernel;           // 0 auto, 1 gather, 2 tile
+  int kernel;           // 0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled
   void* dev;            // single device allocation backing the tables
 };
 
 __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __restrict__ in,
-                                                    c128* __restrict__ out, long long ld,
-                                                    const double* __restrict__ in_scale,
-                                                    c128* __restrict__ copy_out,
-                                                    const int* __restrict__ active) {
-  if (active && *active == 0) return;
+                                                    c128* __restrict__ out, long long ld, ApplyCtl A) {
+  const long long b = blockIdx.y;
+  double s = 1.0;
+  if (A.ctl) {
+    const BasisCtl& c = A.ctl[b];
+    if (!c.active || A.j >= c.target) return;
+    s = c.scale;
+  }
   const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= T.NLR) return;
-  const long long b = blockIdx.y;
+  PairCoef E;
+  load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
   const c128* x = in + b * ld;
   c128* y = out + b * ld;
   const int l = static_cast<int>(t / T.NR);
@@ -67,11 +70,11 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     const int eA = e >> 1, eB = e & 1;
-    const c128 d = cadd(cadd(T.dA[eA * T.NL + l], T.dB[eB * T.NR + r]), T.pdiag[e]);
+    const c128 d = cadd(cadd(T.dA[eA * T.NL + l], T.dB[eB * T.NR + r]), E.d[e]);
     acc[e] = cfma(d, p[e], cmake(0.0, 0.0));
-    acc[e] = cfma(T.cA[e * T.NL + l], p[e ^ 2], acc[e]);
-    acc[e] = cfma(T.cB[e * T.NR + r], p[e ^ 1], acc[e]);
-    acc[e] = cfma(T.pboth[e], p[e ^ 3], acc[e]);
+    acc[e] = cfma(cadd(T.fA[eA * T.NL + l], E.fa[e]), p[e ^ 2], acc[e]);
+    acc[e] = cfma(cadd(T.fB[eB * T.NR + r], E.fb[e]), p[e ^ 1], acc[e]);
+    acc[e] = cfma(E.bo[e], p[e ^ 3], acc[e]);
   }
 
   // left nuclei: couple (eA, m_n) with (e', m'), eB and r unchanged
@@ -98,11 +101,10 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
     acc[3] = cfma(c10, q2, acc[3]); acc[3] = cfma(c11, q3, acc[3]);
   }
 
-  const double s = in_scale ? *in_scale : 1.0;
 #pragma unroll
   for (int e = 0; e < 4; ++e) y[e * S + t] = cscale(acc[e], s);
-  if (copy_out) {
-    c128* z = copy_out + b * ld;
+  if (A.copy) {
+    c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
     for (int e = 0; e < 4; ++e) z[e * S + t] = cscale(p[e], s);
   }
@@ -178,14 +180,7 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   side(nL, ldims, NL, KL, 0, NR, dA, fA, offL, coefL);
   side(nR, rdims, NR, KR, nL, 1, dB, fB, offR, coefR);
 
-  // combine with the electron pair term
-  auto Pel = [&](int row, int col) { return P[row * 4 + col]; };
-  std::vector<c128> cA(4 * NL), cB(4 * NR);
-  for (int e = 0; e < 4; ++e) {
-    const int eA = e >> 1, eB = e & 1;
-    for (long long l = 0; l < NL; ++l) cA[e * NL + l] = cadd(fA[eA * NL + l], Pel(e, e ^ 2));
-    for (long long r = 0; r < NR; ++r) cB[e * NR + r] = cadd(fB[eB * NR + r], Pel(e, e ^ 1));
-  }
+  std::vector<c128> pairv(P, P + 16);
 
   // ---- tiled-kernel tables: the last NC = 2 left axes (spin-1/2) are closed in the tile
   const int NC = 2;
@@ -286,15 +281,11 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   T.KL = KL;
   T.KR = KR;
   T.NLR = NL * NR;
-  for (int e = 0; e < 4; ++e) {
-    T.pdiag[e] = Pel(e, e);
-    T.pboth[e] = Pel(e, e ^ 3);
-  }
   // pack device buffer
   size_t bytes = 0;
   auto reserve = [&](size_t n) { size_t o = bytes; bytes += (n + 255) & ~size_t(255); return o; };
   const size_t o_dA = reserve(dA.size() * 16), o_dB = reserve(dB.size() * 16);
-  const size_t o_cA = reserve(cA.size() * 16), o_cB = reserve(cB.size() * 16);
+  const size_t o_cA = reserve(fA.size() * 16), o_cB = reserve(fB.size() * 16), o_P = reserve(16 * 16);
   const size_t o_offL = reserve(offL.size() * 4), o_offR = reserve(offR.size() * 4);
   const size_t o_coL = reserve(coefL.size() * 16), o_coR = reserve(coefR.size() * 16);
   const size_t o_cC = reserve(coefC.size() * 16);
@@ -309,8 +300,9 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   std::memcpy(host.data() + o_cT, coefT.data(), coefT.size() * 16);
   std::memcpy(host.data() + o_dA, dA.data(), dA.size() * 16);
   std::memcpy(host.data() + o_dB, dB.data(), dB.size() * 16);
-  std::memcpy(host.data() + o_cA, cA.data(), cA.size() * 16);
-  std::memcpy(host.data() + o_cB, cB.data(), cB.size() * 16);
+  std::memcpy(host.data() + o_cA, fA.data(), fA.size() * 16);
+  std::memcpy(host.data() + o_cB, fB.data(), fB.size() * 16);
+  std::memcpy(host.data() + o_P, pairv.data(), 16 * 16);
   std::memcpy(host.data() + o_offL, offL.data(), offL.size() * 4);
   std::memcpy(host.data() + o_offR, offR.data(), offR.size() * 4);
   std::memcpy(host.data() + o_coL, coefL.data(), coefL.size() * 16);
@@ -328,8 +320,9 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   char* base = static_cast<char*>(dev);
   T.dA = reinterpret_cast<const c128*>(base + o_dA);
   T.dB = reinterpret_cast<const c128*>(base + o_dB);
-  T.cA = reinterpret_cast<const c128*>(base + o_cA);
-  T.cB = reinterpret_cast<const c128*>(base + o_cB);
+  T.fA = reinterpret_cast<const c128*>(base + o_cA);
+  T.fB = reinterpret_cast<const c128*>(base + o_cB);
+  T.pair = reinterpret_cast<const c128*>(base + o_P);
   T.offL = reinterpret_cast<const int*>(base + o_offL);
   T.offR = reinterpret_cast<const int*>(base + o_offR);
   T.coefL = reinterpret_cast<const c128*>(base + o_coL);
@@ -347,8 +340,9 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     U.NLR = T.NLR;
     U.dA = T.dA;
     U.dB = T.dB;
-    U.cA = T.cA;
-    U.cB = T.cB;
+    U.fA = T.fA;
+    U.fB = T.fB;
+    U.pair = T.pair;
     U.offR = T.offR;
     U.coefR = T.coefR;
     U.maskR = 0;
@@ -361,10 +355,6 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     U.maskO = reinterpret_cast<const unsigned*>(base + o_fO);
     U.packR = reinterpret_cast<const int*>(base + o_pR);
     U.coefT = reinterpret_cast<const c128*>(base + o_cT);
-    for (int e = 0; e < 4; ++e) {
-      U.pdiag[e] = T.pdiag[e];
-      U.pboth[e] = T.pboth[e];
-    }
   }
   op->dev = dev;
   *out = op;
@@ -398,14 +388,16 @@ __global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
                                                      const c128* __restrict__ ops,
                                                      const long long* __restrict__ cols,
                                                      const c128* __restrict__ in,
-                                                     c128* __restrict__ out, long long ld,
-                                                     const double* __restrict__ in_scale,
-                                                     c128* __restrict__ copy_out,
-                                                     const int* __restrict__ active) {
-  if (active && *active == 0) return;
+                                                     c128* __restrict__ out, long long ld, ApplyCtl A) {
+  const long long b = blockIdx.y;
+  double s = 1.0;
+  if (A.ctl) {
+    const BasisCtl& c = A.ctl[b];
+    if (!c.active || A.j >= c.target) return;
+    s = c.scale;
+  }
   const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const long long b = blockIdx.y;
   const c128* x = in + b * ld;
   c128 acc{0.0, 0.0};
   for (int t = 0; t < nterms; ++t) {
@@ -421,9 +413,8 @@ __global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
     const long long* cc = cols + T.col_offset;
     for (int c = 0; c < T.D; ++c) acc = cfma(row[c], x[base + cc[c]], acc);
   }
-  const double s = in_scale ? *in_scale : 1.0;
   out[b * ld + i] = cscale(acc, s);
-  if (copy_out) copy_out[b * ld + i] = cscale(x[i], s);
+  if (A.copy) A.copy[b * A.copy_ld + i] = cscale(x[i], s);
 }
 
 static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes, const int* taxes,
@@ -504,7 +495,7 @@ static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes
 }
 
 int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int batch, long long ld,
-                      const double* in_scale, c128* copy_out, const int* active, hipStream_t stream) {
+                      const ApplyCtl& A, hipStream_t stream) {
   if (!op) return fail(RPM_E_INVALID, "NULL operator");
   if (batch < 1 || batch > 65535) return fail(RPM_E_INVALID, "batch %d out of range", batch);
   if (ld < op->n) return fail(RPM_E_INVALID, "ld %lld smaller than the dimension %lld", ld, op->n);
@@ -514,18 +505,17 @@ int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int bat
     if (P->use_tile && P->kernel != 1) {
       if (static_cast<long long>(P->tile.ngroups) * batch > 0x7fffffffLL)
         return fail(RPM_E_INVALID, "batch %d too large for one launch", batch);
-      if (P->kernel == 2) return launch_pair_tile(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
-      if (P->kernel == 4) return launch_pair_tile4(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
-      return launch_pair_tile3(P->tile, in, out, batch, ld, in_scale, copy_out, active, stream);
+      if (P->kernel == 2) return launch_pair_tile(P->tile, in, out, batch, ld, A, stream);
+      return launch_pair_tile3(P->tile, in, out, batch, ld, A, stream);
     }
     dim3 grid(ceil_div(P->t.NLR, 256), batch);
-    hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, in_scale,
-                       copy_out, active);
+    hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, A);
   } else {
+    if (A.pairs) return fail(RPM_E_UNSUPPORTED, "per-vector electron matrices need a pair operator");
     const LocalOp* L = static_cast<const LocalOp*>(op->impl);
     dim3 grid(ceil_div(L->n, 256), batch);
     hipLaunchKernelGGL(k_local_apply, grid, dim3(256), 0, stream, L->nterms, L->n, L->terms, L->ops,
-                       L->cols, in, out, ld, in_scale, copy_out, active);
+                       L->cols, in, out, ld, A);
   }
   RPM_LAUNCH_CHECK();
   return RPM_OK;
@@ -587,12 +577,22 @@ extern "C" long long rpm_operator_dimension(const rpm_operator* op) { return op 
 extern "C" int rpm_operator_apply(const rpm_operator* op, const void* d_in, void* d_out, int batch,
                                   long long ld, void* stream) {
   if (d_in == d_out) return fail(RPM_E_INVALID, "d_in and d_out must not overlap");
-  return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld,
-                           nullptr, nullptr, nullptr, static_cast<hipStream_t>(stream));
+  return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld, ApplyCtl{},
+                           static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rpm_operator_apply_pairs(const rpm_operator* op, const void* d_in, void* d_out, int batch,
+                                        long long ld, const void* d_pairs, void* stream) {
+  if (d_in == d_out) return fail(RPM_E_INVALID, "d_in and d_out must not overlap");
+  if (!d_pairs) return fail(RPM_E_INVALID, "rpm_operator_apply_pairs: NULL electron matrices");
+  ApplyCtl A;
+  A.pairs = static_cast<const c128*>(d_pairs);
+  return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld, A,
+                           static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rpm_operator_set_kernel(rpm_operator* op, int kernel) {
-  if (!op || kernel < 0 || kernel > 4) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
+  if (!op || kernel < 0 || kernel > 3) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
   if (op->kind != 1) return kernel >= 2 ? fail(RPM_E_UNSUPPORTED, "tiled kernels need a pair operator") : RPM_OK;
   PairOp* P = static_cast<PairOp*>(op->impl);
   if (kernel >= 2 && !P->use_tile) return fail(RPM_E_UNSUPPORTED, "the operator shape has no tiled kernel");
@@ -605,5 +605,5 @@ extern "C" int rpm_operator_kernel(const rpm_operator* op) {
   if (op->kind != 1) return 9;
   const PairOp* P = static_cast<const PairOp*>(op->impl);
   if (!P->use_tile || P->kernel == 1) return 1;
-  return (P->kernel == 2 || P->kernel == 4) ? P->kernel : 3;
+  return P->kernel == 2 ? 2 : 3;
 }

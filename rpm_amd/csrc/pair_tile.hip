@@ -1,19 +1,19 @@
-// Tiled pair stencil for the full-size two-radical layout (the production
+// Tiled pair stencils for the full-size two-radical layout (the production
 // FAD-TrpH operator and random-tensor systems of the same shape).
 //
 // Layout: psi[(e * NL + l) * NR + r], e = eA*2+eB.  One workgroup owns a tile
 // of all 4 electron states x LG = 2^NC consecutive left indices (the NC
 // lowest left axes, all spin-1/2, are "closed" inside the tile) x all NR right
-// indices; thread r holds the tile column psi[e][l][r] (4*LG amplitudes) in
-// registers and all 4*LG outputs in accumulators:
-//   * diagonal, electron flips and the closed left axes: registers only;
-//   * right-radical neighbours r': read from the tile staged in LDS;
+// indices:
+//   * diagonal and electron flips: registers;
+//   * closed left axes and right-radical neighbours r': the tile, staged in LDS
+//     (k_pair_tile3) or held in registers (k_pair_tile);
 //   * the other left axes (open, KO neighbour offsets): read from global
 //     memory (L2), the same offsets and coefficients for the whole tile.
-// Every coefficient 2x2 electron block carries flags so that structurally
-// zero halves (same-electron or electron-flip) are skipped with uniform
-// branches.  Workgroups are mapped XCD-aware: all tiles of a vector run on one
-// XCD, so the re-read neighbours stay in that XCD's L2.
+// Every 2x2 electron block of nuclear couplings carries a 2-bit mask so that
+// structurally zero halves (same-electron or electron-flip) are skipped with
+// scalar branches.  Workgroups are mapped XCD-aware: all tiles of a vector run
+// on one XCD, so the re-read neighbours stay in that XCD's L2.
 #include <cstring>
 #include <vector>
 
@@ -54,27 +54,43 @@ __device__ inline void mac2x2_right(const c128 C[4], int flags, const c128 q[4],
   }
 }
 
+// XCD-aware mapping of the linear workgroup id onto (vector, group).
+__device__ inline void tile_map(int G, int batch, int& b, int& grp) {
+  const int g = blockIdx.x;
+  if (batch % kXcds == 0) {
+    const int x = g % kXcds, k = g / kXcds;
+    b = (k / G) * kXcds + x;
+    grp = k % G;
+  } else {
+    b = g / G;
+    grp = g % G;
+  }
+}
+
+// Per-vector control: false when vector b is idle in this launch; s = input scale.
+__device__ inline bool tile_begin(const ApplyCtl& A, int b, double& s) {
+  s = 1.0;
+  if (A.ctl) {
+    const BasisCtl& c = A.ctl[b];
+    if (!c.active || A.j >= c.target) return false;
+    s = c.scale;
+  }
+  return true;
+}
+
+// Register-tiled variant: thread r holds the tile column psi[e][l][r] (4*LG
+// amplitudes) and all 4*LG outputs.
 template <int NR, int NC, int KO, int KR>
 __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __restrict__ in, c128* __restrict__ out,
-                                                  long long ld, int batch, const double* __restrict__ in_scale,
-                                                  c128* __restrict__ copy_out, const int* __restrict__ active) {
+                                                  long long ld, int batch, ApplyCtl A) {
   constexpr int LG = 1 << NC;
-  if (active && *active == 0) return;
   __shared__ c128 tile[4 * LG * NR];
-  // XCD-aware mapping of the linear workgroup id onto (vector, group)
-  const int G = T.ngroups;
   int b, grp;
-  {
-    const int g = blockIdx.x;
-    if (batch % kXcds == 0) {
-      const int x = g % kXcds, k = g / kXcds;
-      b = (k / G) * kXcds + x;
-      grp = k % G;
-    } else {
-      b = g / G;
-      grp = g % G;
-    }
-  }
+  tile_map(T.ngroups, batch, b, grp);
+  double s;
+  if (!tile_begin(A, b, s)) return;
+  PairCoef E;
+  load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
   const int r = threadIdx.x;
   const long long S = T.NLR;
   const c128* __restrict__ x = in + b * ld;
@@ -89,40 +105,34 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
       tile[(e * LG + l) * NR + r] = psi[l][e];
     }
 
-  // diagonal and electron flips
   c128 acc[LG][4];
   {
-    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
-    c128 cB[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) cB[e] = T.cB[e * NR + r];
+    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r], fB0 = T.fB[r], fB1 = T.fB[NR + r];
 #pragma unroll
     for (int l = 0; l < LG; ++l) {
       const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
+      const c128 fA0 = T.fA[l0 + l], fA1 = T.fA[T.NL + l0 + l];
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
+        const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), E.d[e]);
         c128 a = cfma(d, psi[l][e], cmake(0.0, 0.0));
-        a = cfma(T.cA[e * T.NL + l0 + l], psi[l][e ^ 2], a);
-        a = cfma(cB[e], psi[l][e ^ 1], a);
-        a = cfma(T.pboth[e], psi[l][e ^ 3], a);
+        a = cfma(cadd((e >> 1) ? fA1 : fA0, E.fa[e]), psi[l][e ^ 2], a);
+        a = cfma(cadd((e & 1) ? fB1 : fB0, E.fb[e]), psi[l][e ^ 1], a);
+        a = cfma(E.bo[e], psi[l][e ^ 3], a);
         acc[l][e] = a;
       }
     }
   }
 
-  // closed left axes: neighbour of l along closed axis j is l ^ (1 << j)
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
 #pragma unroll
     for (int l = 0; l < LG; ++l) {
       const int m = (l >> j) & 1;
-      const c128* C = T.coefC + (j * 2 + m) * 4;
-      mac2x2_left(C, (T.maskC >> (2 * j)) & 3, psi[l ^ (1 << j)], acc[l]);
+      mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, psi[l ^ (1 << j)], acc[l]);
     }
   }
 
-  // open left axes: same offsets/coefficients for the whole tile, data from L2
   {
     const int* __restrict__ ofs = T.ofsO + grp * KO;
     const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
@@ -131,19 +141,17 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
     for (int k = 0; k < KO; ++k) {
       const long long o = static_cast<long long>(l0 + ofs[k]) * NR + r;
       const int f = (fm >> (2 * k)) & 3;
-      const c128* C = cof + k * 4;
 #pragma unroll
       for (int l = 0; l < LG; ++l) {
         c128 q[4];
 #pragma unroll
         for (int e = 0; e < 4; ++e) q[e] = x[e * S + o + static_cast<long long>(l) * NR];
-        mac2x2_left(C, f, q, acc[l]);
+        mac2x2_left(cof + k * 4, f, q, acc[l]);
       }
     }
   }
 
   __syncthreads();
-  // right-radical neighbours from the LDS tile
 #pragma unroll 1
   for (int k = 0; k < KR; ++k) {
     const int r2 = r + T.offR[r * KR + k];
@@ -160,7 +168,6 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
     }
   }
 
-  const double s = in_scale ? *in_scale : 1.0;
   c128* __restrict__ y = out + b * ld;
 #pragma unroll
   for (int l = 0; l < LG; ++l)
@@ -170,8 +177,8 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
       __builtin_nontemporal_store(acc[l][e].re * s, &y[idx].re);
       __builtin_nontemporal_store(acc[l][e].im * s, &y[idx].im);
     }
-  if (copy_out) {
-    c128* z = copy_out + b * ld;
+  if (A.copy) {
+    c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
     for (int l = 0; l < LG; ++l)
 #pragma unroll
@@ -179,32 +186,22 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
   }
 }
 
-// Variant with one (l, r) per thread: 4 outputs per thread, low register
-// pressure and high occupancy; closed left axes and right neighbours from LDS,
-// right-radical coefficients from a (slot, digit) table staged in LDS.
+// LDS-tiled variant (default): one (l, r) per thread, 4 outputs per thread,
+// closed left axes and right neighbours from LDS, right-radical coefficients
+// from a (slot, digit) table staged in LDS.
 template <int NR, int NC, int KO, int KR>
-__global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6, 8))) k_pair_tile3(TileTables T, const c128* __restrict__ in,
-                                                         c128* __restrict__ out, long long ld, int batch,
-                                                         const double* __restrict__ in_scale,
-                                                         c128* __restrict__ copy_out, const int* __restrict__ active) {
+__global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6, 8)))
+k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, long long ld, int batch, ApplyCtl A) {
   constexpr int LG = 1 << NC;
   constexpr int NT = NR * LG;
-  if (active && *active == 0) return;
   __shared__ c128 tile[4 * LG * NR];
   __shared__ c128 coefT[KR * 3 * 4];
-  const int G = T.ngroups;
   int b, grp;
-  {
-    const int g = blockIdx.x;
-    if (batch % kXcds == 0) {
-      const int x = g % kXcds, k = g / kXcds;
-      b = (k / G) * kXcds + x;
-      grp = k % G;
-    } else {
-      b = g / G;
-      grp = g % G;
-    }
-  }
+  tile_map(T.ngroups, batch, b, grp);
+  double s;
+  if (!tile_begin(A, b, s)) return;
+  PairCoef E;
+  load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
   const int tid = threadIdx.x;
   const int l = tid / NR, r = tid - (tid / NR) * NR;
   const long long S = T.NLR;
@@ -223,14 +220,15 @@ __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6
   c128 acc[4];
   {
     const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
-    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
+    const c128 fA0 = T.fA[l0 + l], fA1 = T.fA[T.NL + l0 + l];
+    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r], fB0 = T.fB[r], fB1 = T.fB[NR + r];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
+      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), E.d[e]);
       c128 a = cfma(d, psi[e], cmake(0.0, 0.0));
-      a = cfma(T.cA[e * T.NL + l0 + l], psi[e ^ 2], a);
-      a = cfma(T.cB[e * NR + r], psi[e ^ 1], a);
-      a = cfma(T.pboth[e], psi[e ^ 3], a);
+      a = cfma(cadd((e >> 1) ? fA1 : fA0, E.fa[e]), psi[e ^ 2], a);
+      a = cfma(cadd((e & 1) ? fB1 : fB0, E.fb[e]), psi[e ^ 1], a);
+      a = cfma(E.bo[e], psi[e ^ 3], a);
       acc[e] = a;
     }
   }
@@ -251,7 +249,6 @@ __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6
   }
 
   __syncthreads();
-  // closed left axes from LDS
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     const int l2 = l ^ (1 << j);
@@ -261,7 +258,6 @@ __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6
     for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
     mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, q, acc);
   }
-  // right neighbours from LDS
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
     const int pk = T.packR[r * KR + k];
@@ -275,168 +271,39 @@ __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6
     mac2x2_right(C, (T.maskR >> (2 * k)) & 3, q, acc);
   }
 
-  const double s = in_scale ? *in_scale : 1.0;
   c128* __restrict__ y = out + b * ld;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
     __builtin_nontemporal_store(acc[e].re * s, &y[e * S + own].re);
     __builtin_nontemporal_store(acc[e].im * s, &y[e * S + own].im);
   }
-  if (copy_out) {
-    c128* z = copy_out + b * ld;
+  if (A.copy) {
+    c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
     for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(psi[e], s);
   }
-}
-
-// Variant of k_pair_tile3 that issues every global load of a thread (its own
-// 4 amplitudes and all open-axis neighbours) before any arithmetic, trading
-// occupancy for memory-level parallelism; the left index is made wave-uniform
-// so diagonal / flip / closed-axis coefficients come through scalar loads.
-template <int NR, int NC, int KO, int KR>
-__global__ void __launch_bounds__(NR << NC) k_pair_tile4(TileTables T, const c128* __restrict__ in,
-                                                         c128* __restrict__ out, long long ld, int batch,
-                                                         const double* __restrict__ in_scale,
-                                                         c128* __restrict__ copy_out, const int* __restrict__ active) {
-  constexpr int LG = 1 << NC;
-  constexpr int NT = NR * LG;
-  if (active && *active == 0) return;
-  __shared__ c128 tile[4 * LG * NR];
-  __shared__ c128 coefT[KR * 3 * 4];
-  const int G = T.ngroups;
-  int b, grp;
-  {
-    const int g = blockIdx.x;
-    if (batch % kXcds == 0) {
-      const int x = g % kXcds, k = g / kXcds;
-      b = (k / G) * kXcds + x;
-      grp = k % G;
-    } else {
-      b = g / G;
-      grp = g % G;
-    }
-  }
-  const int tid = threadIdx.x;
-  // NR is a multiple of the wave size, so l is uniform across the wave
-  const int l = __builtin_amdgcn_readfirstlane(tid / NR);
-  const int r = tid - l * NR;
-  const int S = static_cast<int>(T.NLR);
-  const c128* __restrict__ x = in + b * ld;
-  const int l0 = grp * LG;
-  const int own = (l0 + l) * NR + r;
-  const int* __restrict__ ofs = T.ofsO + grp * KO;
-
-  c128 psi[4];
-  c128 q[KO][4];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) psi[e] = x[e * S + own];
-#pragma unroll
-  for (int k = 0; k < KO; ++k) {
-    const int o = own + ofs[k] * NR;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) q[k][e] = x[e * S + o];
-  }
-  for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
-#pragma unroll
-  for (int e = 0; e < 4; ++e) tile[(e * LG + l) * NR + r] = psi[e];
-
-  c128 acc[4];
-  {
-    const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
-    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), T.pdiag[e]);
-      c128 a = cfma(d, psi[e], cmake(0.0, 0.0));
-      a = cfma(T.cA[e * T.NL + l0 + l], psi[e ^ 2], a);
-      a = cfma(T.cB[e * NR + r], psi[e ^ 1], a);
-      a = cfma(T.pboth[e], psi[e ^ 3], a);
-      acc[e] = a;
-    }
-  }
-  {
-    const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
-    const unsigned fm = T.maskO[grp];
-#pragma unroll
-    for (int k = 0; k < KO; ++k) mac2x2_left(cof + k * 4, (fm >> (2 * k)) & 3, q[k], acc);
-  }
-
-  __syncthreads();
-#pragma unroll
-  for (int j = 0; j < NC; ++j) {
-    const int l2 = l ^ (1 << j);
-    const int m = (l >> j) & 1;
-    c128 qq[4];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) qq[e] = tile[(e * LG + l2) * NR + r];
-    mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, qq, acc);
-  }
-#pragma unroll
-  for (int k = 0; k < KR; ++k) {
-    const int pk = T.packR[r * KR + k];
-    const int off = static_cast<short>(pk & 0xffff);
-    const int m = pk >> 16;
-    c128 C[4], qq[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) C[i] = coefT[(k * 3 + m) * 4 + i];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) qq[e] = tile[(e * LG + l) * NR + r + off];
-    mac2x2_right(C, (T.maskR >> (2 * k)) & 3, qq, acc);
-  }
-
-  const double s = in_scale ? *in_scale : 1.0;
-  c128* __restrict__ y = out + b * ld;
-#pragma unroll
-  for (int e = 0; e < 4; ++e) {
-    __builtin_nontemporal_store(acc[e].re * s, &y[e * S + own].re);
-    __builtin_nontemporal_store(acc[e].im * s, &y[e * S + own].im);
-  }
-  if (copy_out) {
-    c128* z = copy_out + b * ld;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(psi[e], s);
-  }
-}
-
-int launch_pair_tile4(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                      c128* copy_out, const int* active, hipStream_t stream) {
-  const int blocks = T.ngroups * batch;
-  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
-    hipLaunchKernelGGL((k_pair_tile4<192, 2, 7, 8>), dim3(blocks), dim3(192 * 4), 0, stream, T, in, out, ld, batch,
-                       in_scale, copy_out, active);
-  } else {
-    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
-  }
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
-}
-
-int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                      c128* copy_out, const int* active, hipStream_t stream) {
-  const int blocks = T.ngroups * batch;
-  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
-    hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8>), dim3(blocks), dim3(192 * 4), 0, stream, T, in, out, ld, batch,
-                       in_scale, copy_out, active);
-  } else {
-    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
-  }
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
-}
-
-int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                     c128* copy_out, const int* active, hipStream_t stream) {
-  const int blocks = T.ngroups * batch;
-  if (T.NR == 192 && T.NC == 2 && T.KO == 7 && T.KR == 8) {
-    hipLaunchKernelGGL((k_pair_tile<192, 2, 7, 8>), dim3(blocks), dim3(192), 0, stream, T, in, out, ld, batch,
-                       in_scale, copy_out, active);
-  } else {
-    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
-  }
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
 }
 
 bool pair_tile_supported(int NR, int NC, int KO, int KR) { return NR == 192 && NC == 2 && KO == 7 && KR == 8; }
+
+int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                     hipStream_t stream) {
+  if (!pair_tile_supported(T.NR, T.NC, T.KO, T.KR))
+    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  hipLaunchKernelGGL((k_pair_tile<192, 2, 7, 8>), dim3(T.ngroups * batch), dim3(192), 0, stream, T, in, out, ld,
+                     batch, A);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                      hipStream_t stream) {
+  if (!pair_tile_supported(T.NR, T.NC, T.KO, T.KR))
+    return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in, out, ld,
+                     batch, A);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
 
 }  // namespace rpm

@@ -12,10 +12,11 @@ constexpr int kFlipE = 2;        // ... a nonzero electron-flip half
 struct TileTables {
   int NL, NR, NC, KO, KR, ngroups;
   long long NLR;
-  const c128* dA;                  // [2][NL]
+  const c128* dA;                  // [2][NL] diagonal from the left nuclei, index eA*NL + l
   const c128* dB;                  // [2][NR]
-  const c128* cA;                  // [4][NL]
-  const c128* cB;                  // [4][NR]
+  const c128* fA;                  // [2][NL] eA flip from the left nuclei (same nuclear index)
+  const c128* fB;                  // [2][NR]
+  const c128* pair;                // [16]    the operator's own 4x4 electron matrix
   const int* offR;                 // [NR][KR]
   const c128* coefR;               // [NR][KR][4]
   unsigned maskR;                  // 2 bits per right slot k: bit 2k same-electron, 2k+1 electron-flip
@@ -26,16 +27,12 @@ struct TileTables {
   const unsigned* maskO;           // [ngroups]  2 bits per open neighbour k
   const int* packR;                // [NR][KR]  (r2 - r) & 0xffff | m << 16 : neighbour offset and digit
   const c128* coefT;               // [KR][3][4] right coefficients by (slot, digit m)
-  c128 pdiag[4];
-  c128 pboth[4];
 };
 
-int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                     c128* copy_out, const int* active, hipStream_t stream);
+int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                     hipStream_t stream);
+int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                      hipStream_t stream);
 bool pair_tile_supported(int NR, int NC, int KO, int KR);
-int launch_pair_tile4(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                      c128* copy_out, const int* active, hipStream_t stream);
-int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const double* in_scale,
-                      c128* copy_out, const int* active, hipStream_t stream);
 
 }  // namespace rpm

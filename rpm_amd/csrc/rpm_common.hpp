@@ -112,10 +112,48 @@ struct rpm_operator {
 };
 
 namespace rpm {
-// Apply H with an optional device scale s (out = s * H in) and an optional copy
-// of the scaled input (copy_out[x] = s * in[x]) written for every x.
-// `active` (device int, nullable): when it points to 0 the launch does nothing.
+
+// Per-vector control block of a Krylov basis under construction.
+struct BasisCtl {
+  int active;       // 0 once the basis broke down (or the vector is idle)
+  int size;         // basis size built so far / final
+  int target;       // requested basis size
+  int pad;
+  double scale;     // 1/norm of the vector the next matvec normalises
+  double pad2;
+};
+
+// Options of one batched H·psi launch (all nullable / off by default):
+//  * ctl: per-vector control; vector b is skipped when !ctl[b].active or
+//    j >= ctl[b].target, and its input is scaled by ctl[b].scale
+//    (out = s * H in);
+//  * copy: copy[b * copy_ld + x] = s * in[b][x] for every x (the normalised
+//    Krylov vector, written as a side product of the matvec);
+//  * pairs: per-vector 4x4 electron matrix (16 complex per vector, row
+//    (eA,eB)) replacing the operator's own electron-pair/Zeeman matrix.
+struct ApplyCtl {
+  const BasisCtl* ctl = nullptr;
+  int j = 0;
+  c128* copy = nullptr;
+  long long copy_ld = 0;
+  const c128* pairs = nullptr;
+};
+
+// Electron-pair coefficients for vector b: diagonal, eA flip, eB flip, both flip.
+struct PairCoef {
+  c128 d[4], fa[4], fb[4], bo[4];
+};
+
+__device__ inline void load_pair_coef(const c128* __restrict__ P, PairCoef& C) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    C.d[e] = P[e * 4 + e];
+    C.fa[e] = P[e * 4 + (e ^ 2)];
+    C.fb[e] = P[e * 4 + (e ^ 1)];
+    C.bo[e] = P[e * 4 + (e ^ 3)];
+  }
+}
+
 int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int batch, long long ld,
-                      const double* in_scale, c128* copy_out, const int* active,
-                      hipStream_t stream);
+                      const ApplyCtl& A, hipStream_t stream);
 }  // namespace rpm
