@@ -132,13 +132,47 @@ def batch_probe(h, args) -> int:
         assert float(np.abs(r.observable_values - res[i].observable_values).max()) < 1e-12
     horizon = float(times[-1])
     batched_rate = B * horizon / t_batch
-    plain_rate = nplain * horizon / t_plain
+    plain_rate = nplain * horizon / t_plain if nplain else float("nan")
     matvecs = sum(r.stats.matvecs for r in res)
     print(json.dumps({"mode": "batch", "trajectories": B, "horizon_us": horizon, "seconds_batched": t_batch,
                       "trajectory_us_per_s_batched": batched_rate, "plain_sample": nplain,
                       "trajectory_us_per_s_plain": plain_rate, "speedup_vs_plain": batched_rate / plain_rate,
                       "yields_per_s_batched_equiv": batched_rate / cfg.maximum_time_us,
                       "matvecs": matvecs, "rounds": bp.rounds}), flush=True)
+    return 0
+
+
+def grad_probe(h, args) -> int:
+    """Yield-only vs yield + adjoint gradient for a batch of production trajectories over a short horizon."""
+    import numpy as np
+    import torch
+    from rpm_amd import adjoint, batch, core
+    cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+    times = np.arange(int(round(args.horizon / 0.00125)) + 1) * 0.00125
+    B = args.batch
+    thetas = list(np.linspace(10.0, 170.0, B))
+    phis = [0.0] * B
+    fields = [50.0] * B
+    states = np.stack([core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i) for i in range(B)])
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(h, cfg, probe_horizon=float(times[-1])))
+    build = lambda t, f, p: core.build_hamiltonian(t, f, phi_deg=p)  # noqa: E731
+    warm = adjoint.FieldYield(h, build, bp, states[:1], times[:9], cfg.lifetime_us)
+    warm.forward(thetas[:1], phis[:1], fields[:1])
+    warm.backward()
+    prob = adjoint.FieldYield(h, build, bp, states, times, cfg.lifetime_us)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y = prob.forward(thetas, phis, fields)
+    torch.cuda.synchronize()
+    t_yield = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    fgrad, _, rec = prob.backward()
+    torch.cuda.synchronize()
+    t_grad = time.perf_counter() - t0
+    print(json.dumps({"mode": "grad", "trajectories": B, "horizon_us": float(times[-1]), "intervals": times.size - 1,
+                      "seconds_yield": t_yield, "seconds_adjoint": t_grad,
+                      "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield,
+                      "psi0_recovery_error": rec, "grad_theta_first": float(fgrad[0, 0])}), flush=True)
     return 0
 
 
@@ -152,7 +186,7 @@ def main() -> int:
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
-    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch"),
+    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -175,6 +209,8 @@ def main() -> int:
         return propagate_probe(h, args)
     if args.mode == "batch":
         return batch_probe(h, args)
+    if args.mode == "grad":
+        return grad_probe(h, args)
     h._device_op().set_kernel(args.kernel)
     n = h.dimension
     B = args.batch

@@ -32,6 +32,13 @@
  *   rpm_krylov_tasks           _candidate / evaluate_projected         R/rpm_statevector_engine.py:690-715,531-538
  *   rpm_singlet_gram_batch     SingletPopulationObservable.project_basis R/rpm_statevector_engine.py:521-529
  *   rpm_krylov_combine_batch   endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
+ *
+ * Segmented propagation and adjoint gradients:
+ *   rpm_axis_apply             local_unitary_apply                     R/poster_reproduction/core.py:285-292
+ *   rpm_operator_apply_affine  Chebyshev/Clenshaw step of the adjoint sweep (no reference counterpart: the
+ *                              reference has no gradients; dH/dp is its Zeeman form R/poster_reproduction/core.py:186-194)
+ *   rpm_singlet_kick           costate source 2 W_k P psi_k of the Simpson yield R/rpm_quantum_needle_jd_sweeps.py:418-420
+ *   rpm_cross_density          electron 4x4 of <lambda| dH/dp |psi> for electron-only dH/dp (no reference counterpart)
  */
 #ifndef RPM_AMD_H
 #define RPM_AMD_H
@@ -170,6 +177,50 @@ int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k, int batch
 int rpm_krylov_combine_batch(const void* d_basis, int m, const int* d_k, int batch, long long n,
                              const void* d_c, int c_stride, const double* d_scale, void* d_out,
                              long long out_ld, void* stream);
+
+/*
+ * d_out[b] = s * H_b d_in[b] + a_in * d_in[b] + (c0_re + i c0_im) * d_x0[b] + c1 * d_x1[b]
+ * (pair operators; H_b as in rpm_operator_apply_pairs when d_pairs is given).  d_x0 / d_x1
+ * are nullable; d_x1 may be d_out (in place), d_in may alias none of the others.
+ */
+int rpm_operator_apply_affine(const rpm_operator* op, const void* d_in, void* d_out, int batch, long long ld,
+                              const void* d_pairs, double s, double a_in, const void* d_x0, double c0_re,
+                              double c0_im, const void* d_x1, double c1, void* stream);
+
+/* d_acc[b*16 + e*4 + f] += weight * sum_n conj(a_b[e*nn + n]) b_b[f*nn + n]  (e, f = eA*2 + eB). */
+size_t rpm_cross_density_workspace_bytes(long long nn, int batch);
+int rpm_cross_density(const void* d_a, const void* d_b, int batch, long long ld, long long nn, double weight,
+                      void* d_acc, void* d_workspace, size_t workspace_bytes, void* stream);
+
+/* d_g[b] += coef * P d_psi[b] with P the singlet projector |S><S| (x) 1 (N = 4 nn). */
+int rpm_singlet_kick(const void* d_psi, void* d_g, int batch, long long ld, long long nn, double coef,
+                     void* stream);
+
+/* d_out[b] = (1 (x) U (x) 1) d_in[b]: U (d x d complex, row-major) on tensor axis `axis` of the
+ * C-order state with dimensions dims[naxes]; d = dims[axis] <= 16. */
+int rpm_axis_apply(int naxes, const int* dims, int axis, const void* d_U, const void* d_in, void* d_out,
+                   int batch, long long ld, void* stream);
+
+/*
+ * Adjoint sweep of the Simpson singlet yield Phi = sum_k W_k <psi_k|P|psi_k> on the output grid
+ * t_0..t_M (no reference counterpart; the reference computes Phi only,
+ * R/rpm_quantum_needle_jd_sweeps.py:418-420).  Runs backward from d_psi = psi_M (overwritten
+ * with psi_0): per interval [t_{k-1}, t_k] the costate gets g += 2 W_k P psi_k, then psi and g
+ * are carried back by the Chebyshev expansion of exp(+i H h) (X = (H - shift)/half_width,
+ * coefficients c_n = exp(-i shift h) a_n(half_width h), Clenshaw recurrences), and
+ *   d_acc[slot[k-1]][b] += sum_n w_n / half_width * C(nu_{n+1}, T_n(X) psi_{k-1})
+ * (w_0 = 1, w_n = 2, C as in rpm_cross_density, nu the Clenshaw vectors of g), so that
+ * dPhi/dp = Re sum_{e,f} (dH_b/dp)[e][f] acc[e][f] for every electron-only dH/dp.
+ * h_int[M] interval lengths (host), weights[M+1] (host), cls[M] interval class, nterms[ncls]
+ * series lengths K (2..kmax), coef[ncls][kmax+1] complex c_n (host, interleaved), seg[M]
+ * electron-matrix segment (d_pairs: nseg x batch x 16 complex, nullable), slot[M] accumulator
+ * slot (d_acc: nslot x batch x 16 complex, accumulated into).
+ */
+size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch, int kmax);
+int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nint, const double* h_int,
+                      const double* weights, double shift, double half_width, int ncls, const int* nterms,
+                      int kmax, const double* coef, const int* cls, const void* d_pairs, const int* seg,
+                      void* d_acc, const int* slot, void* d_workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Embedded error estimates for a ladder of candidate dimensions.

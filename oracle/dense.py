@@ -62,3 +62,33 @@ def dense_singlet_yield_closed_form(h, state, tau, t_final) -> float:
     integral = k * (1.0 - np.exp(-decay * t_final)) / decay
     terms = (c[:, None] * c.conj()[None, :]) * pm.T * integral
     return float(terms.sum().real)
+
+
+def dense_grid_yield(pieces, state, times, tau):
+    """Simpson yield on an output grid with exact piecewise-constant dynamics (test oracle).
+
+    ``pieces``: [(t_end, H_dense), ...] covering [0, times[-1]]; P_S(t_k) =
+    ||(psi[01] - psi[10])/sqrt2||^2 of exp(-i H t) psi (eigendecomposition per
+    piece); Phi = scipy.integrate.simpson(exp(-t/tau) P_S / tau, x=t)
+    (R/rpm_quantum_needle_jd_sweeps.py:418-420).  Returns (Phi, P_S).
+    """
+    import scipy.integrate
+    t = np.asarray(times, dtype=float)
+    psi = np.asarray(state, dtype=complex).copy()
+    n = psi.size
+    pops = np.empty(t.size)
+    start = 0.0
+    k = 0
+    for t_end, h in pieces:
+        w, q = np.linalg.eigh(h)
+        c = q.conj().T @ psi
+        while k < t.size and t[k] <= t_end + 1e-12:
+            v = q @ (c * np.exp(-1.0j * w * (t[k] - start)))
+            s = (v[n // 4:n // 2] - v[n // 2:3 * n // 4]) / math.sqrt(2.0)
+            pops[k] = float(np.vdot(s, s).real)
+            k += 1
+        psi = q @ (c * np.exp(-1.0j * w * (t_end - start)))
+        start = t_end
+    if k != t.size:
+        raise ValueError("pieces do not cover the output grid")
+    return float(scipy.integrate.simpson(np.exp(-t / tau) * pops / tau, x=t)), pops

@@ -102,6 +102,7 @@ class BatchedLanczosPropagator:
         self.minimum_step_us = minimum_step_us
         self.breakdown_tolerance = breakdown_tolerance
         self.rounds = 0
+        self.final_states = None     # (B, N) device tensor of the last propagate call
 
     @classmethod
     def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
@@ -411,6 +412,7 @@ class BatchedLanczosPropagator:
                     for b in group:
                         resume(b, abs(float(nv[b]) - 1.0))
 
+        self.final_states = states
         out = []
         for b in range(B):
             values, bounds, errors, stats, record = results[b]

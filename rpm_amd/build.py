@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "librpm_amd.so"
-SOURCES = ("errors.cpp", "operators.hip", "pair_tile.hip", "lanczos.hip", "krylov.hip")
+SOURCES = ("errors.cpp", "operators.hip", "pair_tile.hip", "lanczos.hip", "krylov.hip", "adjoint.hip")
 ARCH = "gfx950"
 
 

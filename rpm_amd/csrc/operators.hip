@@ -102,7 +102,7 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
   }
 
 #pragma unroll
-  for (int e = 0; e < 4; ++e) y[e * S + t] = cscale(acc[e], s);
+  for (int e = 0; e < 4; ++e) y[e * S + t] = apply_epilogue(A, acc[e], p[e], s, b, ld, e * S + t);
   if (A.copy) {
     c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
@@ -512,6 +512,7 @@ int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int bat
     hipLaunchKernelGGL(k_pair_apply, grid, dim3(256), 0, stream, P->t, in, out, ld, A);
   } else {
     if (A.pairs) return fail(RPM_E_UNSUPPORTED, "per-vector electron matrices need a pair operator");
+    if (A.affine) return fail(RPM_E_UNSUPPORTED, "the affine epilogue needs a pair operator");
     const LocalOp* L = static_cast<const LocalOp*>(op->impl);
     dim3 grid(ceil_div(L->n, 256), batch);
     hipLaunchKernelGGL(k_local_apply, grid, dim3(256), 0, stream, L->nterms, L->n, L->terms, L->ops,
@@ -587,6 +588,26 @@ extern "C" int rpm_operator_apply_pairs(const rpm_operator* op, const void* d_in
   if (!d_pairs) return fail(RPM_E_INVALID, "rpm_operator_apply_pairs: NULL electron matrices");
   ApplyCtl A;
   A.pairs = static_cast<const c128*>(d_pairs);
+  return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld, A,
+                           static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rpm_operator_apply_affine(const rpm_operator* op, const void* d_in, void* d_out, int batch,
+                                         long long ld, const void* d_pairs, double s, double a_in, const void* d_x0,
+                                         double c0_re, double c0_im, const void* d_x1, double c1, void* stream) {
+  if (d_in == d_out || (d_x0 && d_x0 == d_in) || (d_x1 && d_x1 == d_in))
+    return fail(RPM_E_INVALID, "rpm_operator_apply_affine: the input may not alias the output or x0/x1");
+  if (d_x0 && d_x0 == d_out) return fail(RPM_E_INVALID, "rpm_operator_apply_affine: x0 may not alias the output");
+  if (op && op->kind != 1) return fail(RPM_E_UNSUPPORTED, "the affine epilogue needs a pair operator");
+  ApplyCtl A;
+  A.pairs = static_cast<const c128*>(d_pairs);
+  A.affine = 1;
+  A.aff_s = s;
+  A.aff_in = a_in;
+  A.x0 = static_cast<const c128*>(d_x0);
+  A.aff_c0 = cmake(c0_re, c0_im);
+  A.x1 = static_cast<const c128*>(d_x1);
+  A.aff_c1 = c1;
   return operator_apply_ex(op, static_cast<const c128*>(d_in), static_cast<c128*>(d_out), batch, ld, A,
                            static_cast<hipStream_t>(stream));
 }

@@ -174,8 +174,9 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const long long idx = e * S + static_cast<long long>(l0 + l) * NR + r;
-      __builtin_nontemporal_store(acc[l][e].re * s, &y[idx].re);
-      __builtin_nontemporal_store(acc[l][e].im * s, &y[idx].im);
+      const c128 v = apply_epilogue(A, acc[l][e], psi[l][e], s, b, ld, idx);
+      __builtin_nontemporal_store(v.re, &y[idx].re);
+      __builtin_nontemporal_store(v.im, &y[idx].im);
     }
   if (A.copy) {
     c128* z = A.copy + b * A.copy_ld;
@@ -274,8 +275,9 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   c128* __restrict__ y = out + b * ld;
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    __builtin_nontemporal_store(acc[e].re * s, &y[e * S + own].re);
-    __builtin_nontemporal_store(acc[e].im * s, &y[e * S + own].im);
+    const c128 v = apply_epilogue(A, acc[e], psi[e], s, b, ld, e * S + own);
+    __builtin_nontemporal_store(v.re, &y[e * S + own].re);
+    __builtin_nontemporal_store(v.im, &y[e * S + own].im);
   }
   if (A.copy) {
     c128* z = A.copy + b * A.copy_ld;

@@ -137,7 +137,33 @@ struct ApplyCtl {
   c128* copy = nullptr;
   long long copy_ld = 0;
   const c128* pairs = nullptr;
+  // affine epilogue (pair operators; Chebyshev / Clenshaw recurrences):
+  //   out = aff_s * (H in) + aff_in * in + aff_c0 * x0 + aff_c1 * x1
+  // with x0, x1 (nullable) per vector at b * ld like in and out.
+  int affine = 0;
+  double aff_s = 1.0;
+  double aff_in = 0.0;
+  c128 aff_c0 = {0.0, 0.0};
+  double aff_c1 = 0.0;
+  const c128* x0 = nullptr;
+  const c128* x1 = nullptr;
 };
+
+// Output value of one amplitude: s * (H in) normally, the affine form when requested.
+__device__ inline c128 apply_epilogue(const ApplyCtl& A, c128 hx, c128 xin, double s, long long b, long long ld,
+                                      long long idx) {
+  if (!A.affine) return cscale(hx, s);
+  c128 y = cscale(hx, A.aff_s);
+  y.re = fma(A.aff_in, xin.re, y.re);
+  y.im = fma(A.aff_in, xin.im, y.im);
+  if (A.x0) y = cfma(A.aff_c0, A.x0[b * ld + idx], y);
+  if (A.x1) {
+    const c128 v = A.x1[b * ld + idx];
+    y.re = fma(A.aff_c1, v.re, y.re);
+    y.im = fma(A.aff_c1, v.im, y.im);
+  }
+  return y;
+}
 
 // Electron-pair coefficients for vector b: diagonal, eA flip, eB flip, both flip.
 struct PairCoef {
