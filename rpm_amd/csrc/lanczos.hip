@@ -1,6 +1,13 @@
-// Lanczos basis with classical Gram-Schmidt applied twice (CGS2), norms,
-// singlet projection of a Krylov basis and the basis-combination update, for
-// one trajectory or a batch of trajectories (grid.y = trajectory).
+// Lanczos basis with full reorthogonalisation, norms, singlet projection of a
+// Krylov basis and the basis-combination update, for one trajectory or a batch
+// of trajectories (grid.y = trajectory).
+//
+// Orthogonalisation of w = H v_j (two projections, as classical Gram-Schmidt
+// applied twice): the three-term step w - beta_{j-1} v_{j-1} - alpha_j v_j
+// (beta subtracted inside the matvec, alpha_j = <v_j, H v_j> from the matvec's
+// partial sums) and then one full pass w <- w - V (V^H w).  That is two reads
+// of V per vector.  The reference subtracts the same three terms and then
+// projects twice more; the bases agree to round-off.
 //
 // Follows R/rpm_statevector_engine.py:651-688 (basis), 492-497 (singlet
 // population), 521-529 (projected observable), 904 (endpoint update).
@@ -109,6 +116,7 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
       ctl[b].size = 0;
       ctl[b].target = -1;
       ctl[b].scale = 0.0;
+      ctl[b].beta_prev = 0.0;
     } else {
       norm_out[b] = nrm;
       const bool live = nrm > 0.0;
@@ -116,6 +124,7 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
       ctl[b].size = live ? count : 0;
       ctl[b].target = live ? count : 0;
       ctl[b].scale = live ? 1.0 / nrm : 0.0;
+      ctl[b].beta_prev = 0.0;
     }
   }
 }
@@ -137,6 +146,18 @@ __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict
     h[b * hstride + c] = s;
     if (alpha && c == alpha_col) alpha[b * astride + j] = s.re;
   }
+}
+
+// alpha[b*astride + j] = Re sum_u dots[b][u] (the matvec's per-unit partials of <v_j, H v_j>).
+__global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dots, int units, double* __restrict__ alpha,
+                                                    int astride, int j, const BasisCtl* ctl) {
+  const int b = blockIdx.x;
+  if (!basis_live(ctl, b, j, 0)) return;
+  __shared__ c128 sm[kThreads / kWave];
+  c128 s{0.0, 0.0};
+  for (int i = threadIdx.x; i < units; i += kThreads) s = cadd(s, dots[static_cast<long long>(b) * units + i]);
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) alpha[b * astride + j] = s.re;
 }
 
 // w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2.
@@ -170,14 +191,15 @@ __global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V,
   if (threadIdx.x == 0) norm_partial[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
 }
 
-// Fused CGS pass over an LDS-staged chunk of the basis: optionally
-// w <- w - V h (update), then partial[c][blk] = sum_x conj(V_c[x]) w[x] for
-// all c < k.  Each block walks kGsPer chunks of kGsChunk elements; V is read
-// from HBM once per pass.  k <= kMaxBasis.
+// Fused pass over an LDS-staged chunk of the basis: finishes the three-term
+// step w <- w - alpha_j V_j (column j = k-1 of the chunk), then
+// partial[c][blk] = sum_x conj(V_c[x]) w[x] for all c < k.  Each block walks
+// kGsPer chunks of kGsChunk elements; V is read from HBM once per pass.
+// k <= kMaxBasis.
 constexpr int kGsChunk = 32;
 constexpr int kGsPer = 4;
 __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, long long vstride, long long n, int k,
-                                                 const c128* __restrict__ h, int hstride, c128* __restrict__ W,
+                                                 const double* __restrict__ alpha, int astride, c128* __restrict__ W,
                                                  c128* __restrict__ partial, long long pstride, int j, int need,
                                                  const BasisCtl* ctl) {
   const int b = blockIdx.y;
@@ -185,12 +207,11 @@ __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, lon
   __shared__ c128 vs[kMaxBasis][kGsChunk + 1];
   __shared__ c128 ws[kGsChunk];
   __shared__ c128 red[256];
-  __shared__ c128 hs[kMaxBasis];
   const c128* Vb = V + b * vstride;
   c128* w = W + b * n;
   const int tid = threadIdx.x;
   const int nblk = gridDim.x;
-  if (h && tid < k) hs[tid] = h[b * hstride + tid];
+  const double a = alpha ? alpha[b * astride + j] : 0.0;
   const int dc = tid % 64, dq = tid / 64;   // dot mapping: column dc, 8-element part dq
   c128 dot{0.0, 0.0};
   for (int it = 0; it < kGsPer; ++it) {
@@ -204,25 +225,16 @@ __global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, lon
     }
     if (tid < kGsChunk) {
       const long long x = x0 + tid;
-      ws[tid] = x < n ? w[x] : cmake(0.0, 0.0);
+      c128 wv = x < n ? w[x] : cmake(0.0, 0.0);
+      if (alpha) {
+        const c128 v = x < n ? Vb[static_cast<long long>(k - 1) * n + x] : cmake(0.0, 0.0);
+        wv.re = fma(-a, v.re, wv.re);
+        wv.im = fma(-a, v.im, wv.im);
+        if (x < n) w[x] = wv;
+      }
+      ws[tid] = wv;
     }
     __syncthreads();
-    if (h) {
-      const int i = tid / 8, part = tid % 8;   // 8 threads per element, each over k/8 columns
-      c128 s{0.0, 0.0};
-      for (int c = part; c < k; c += 8) s = cfma(vs[c][i], hs[c], s);
-      red[tid] = s;
-      __syncthreads();
-      if (part == 0) {
-        c128 t = red[tid];
-        for (int q = 1; q < 8; ++q) t = cadd(t, red[tid + q]);
-        const c128 nw = csub(ws[i], t);
-        ws[i] = nw;
-        const long long x = x0 + i;
-        if (x < n) w[x] = nw;
-      }
-      __syncthreads();
-    }
     if (dc < k) {
 #pragma unroll
       for (int i = dq * 8; i < dq * 8 + 8; ++i) dot = cfmaconj(vs[dc][i], ws[i], dot);
@@ -255,6 +267,7 @@ __global__ void __launch_bounds__(kThreads) k_finish_beta(const double* __restri
       ctl[b].size = j + 1;
     } else {
       ctl[b].scale = 1.0 / bj;
+      ctl[b].beta_prev = bj;
     }
   }
 }
@@ -410,6 +423,8 @@ extern "C" int rpm_singlet_population(const void* d_x, long long nn, int batch, 
 
 namespace {
 struct BasisWs {
+  c128* dots;        // [batch][units] matvec partials of alpha_j
+  int units;
   c128* w[2];        // [batch][n] each
   c128* partial;     // [batch][m][nblk]
   long long pstride;
@@ -421,7 +436,7 @@ struct BasisWs {
   size_t bytes;
 };
 
-BasisWs basis_layout(char* base, long long n, int m, int batch) {
+BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   const int nchunk = ceil_div(n, kChunk);
   const int gsb = ceil_div(n, kGsChunk * kGsPer);
   const int nupd = ceil_div(n, kUpdChunk);
@@ -435,6 +450,7 @@ BasisWs basis_layout(char* base, long long n, int m, int batch) {
   const size_t o_w0 = take(B * n * 16), o_w1 = take(B * n * 16);
   const size_t o_p = take(B * m * gsb * 16), o_h = take(B * m * 16), o_np = take(B * nupd * 8);
   const size_t o_sq = take(B * nchunk * 8), o_ctl = take(B * sizeof(BasisCtl));
+  const size_t o_dots = take(B * static_cast<size_t>(units) * 16);
   BasisWs w;
   w.w[0] = reinterpret_cast<c128*>(base + o_w0);
   w.w[1] = reinterpret_cast<c128*>(base + o_w1);
@@ -445,6 +461,8 @@ BasisWs basis_layout(char* base, long long n, int m, int batch) {
   w.npart = reinterpret_cast<double*>(base + o_np);
   w.sq = reinterpret_cast<double*>(base + o_sq);
   w.ctl = reinterpret_cast<BasisCtl*>(base + o_ctl);
+  w.dots = reinterpret_cast<c128*>(base + o_dots);
+  w.units = units;
   w.bytes = off;
   return w;
 }
@@ -456,7 +474,7 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   const long long n = op->n;
   if (max_target > m || max_target < 1) return fail(RPM_E_INVALID, "target %d outside 1..%d", max_target, m);
   if (m > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
-  BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch);
+  BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch, apply_units(op));
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "Lanczos workspace %zu < %zu", ws_bytes, w.bytes);
   const int nchunk = ceil_div(n, kChunk);
   const int nupd = ceil_div(n, kUpdChunk);
@@ -468,40 +486,32 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   for (int j = 0; j < count; ++j) {
     const c128* in = j == 0 ? states : w.w[j % 2];
     c128* out = w.w[(j + 1) % 2];
+    // matvec with the normalisation, the copy V[j] = s in, the three-term subtraction
+    // - beta_{j-1} V[j-1] and the per-unit partials of alpha_j = <v_j, H v_j>
     ApplyCtl A;
     A.ctl = w.ctl;
     A.j = j;
     A.copy = V + static_cast<long long>(j) * n;
     A.copy_ld = vstride;
     A.pairs = pairs;
+    A.prev = j > 0 ? V + static_cast<long long>(j - 1) * n : nullptr;
+    A.prev_ld = vstride;
+    A.dots = w.dots;
+    A.dots_ld = w.units;
     int rc = operator_apply_ex(op, in, out, batch, j == 0 ? sld : n, A, s);
     if (rc) return rc;
-    if (j == count - 1) {
-      // last vector of the longest basis: only alpha_j = Re <v_j, H v_j> is needed
-      hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V + static_cast<long long>(j) * n, vstride,
-                         n, 1, nullptr, m, out, w.partial, w.pstride, j, 0, w.ctl);
-      RPM_LAUNCH_CHECK();
-      hipLaunchKernelGGL(k_reduce_cols, dim3(1, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m,
-                         alpha, m, 0, j, 0, w.ctl);
-      RPM_LAUNCH_CHECK();
-      break;
-    }
+    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, alpha, m, j, w.ctl);
+    RPM_LAUNCH_CHECK();
+    if (j == count - 1) break;
     const int k = j + 1;
-    // pass 1: h1 = V^H w, alpha_j = Re h1[j] (a basis whose target is j + 1 stops here)
-    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, nullptr, m, out, w.partial,
-                       w.pstride, j, 0, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m, alpha,
-                       m, j, j, 0, w.ctl);
-    RPM_LAUNCH_CHECK();
-    // pass 2: w -= V h1 and h2 = V^H w from the same staged chunk
-    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, w.h, m, out, w.partial,
+    // pass 1: w -= alpha_j V[j], h = V^H w
+    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, alpha, m, out, w.partial,
                        w.pstride, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m,
                        nullptr, m, 0, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
-    // pass 3: w -= V h2, ||w||
+    // pass 2: w -= V h, ||w||
     hipLaunchKernelGGL(k_update, dim3(nupd, batch), dim3(kThreads), 0, s, V, vstride, n, k, w.h, m, out, w.npart, j,
                        w.ctl);
     RPM_LAUNCH_CHECK();
@@ -516,12 +526,12 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
 
 extern "C" size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m) {
   if (!op || m < 1) return 0;
-  return basis_layout(nullptr, op->n, m, 1).bytes;
+  return basis_layout(nullptr, op->n, m, 1, apply_units(op)).bytes;
 }
 
 extern "C" size_t rpm_lanczos_batch_workspace_bytes(const rpm_operator* op, int m, int batch) {
   if (!op || m < 1 || batch < 1) return 0;
-  return basis_layout(nullptr, op->n, m, batch).bytes;
+  return basis_layout(nullptr, op->n, m, batch, apply_units(op)).bytes;
 }
 
 extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, void* d_basis,
@@ -532,7 +542,7 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
   const long long n = op->n;
   const int count = static_cast<int>(target < n ? target : n);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1);
+  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1, apply_units(op));
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
   return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
                      static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,

@@ -52,8 +52,9 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
     if (!c.active || A.j >= c.target) return;
     s = c.scale;
   }
-  const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= T.NLR) return;
+  const long long t0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool valid = t0 < T.NLR;
+  const long long t = valid ? t0 : 0;   // idle threads recompute amplitude 0 and store nothing
   PairCoef E;
   load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
   const c128* x = in + b * ld;
@@ -101,12 +102,27 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
     acc[3] = cfma(c10, q2, acc[3]); acc[3] = cfma(c11, q3, acc[3]);
   }
 
+  const double bprev = A.prev ? A.ctl[b].beta_prev : 0.0;
+  c128 dot{0.0, 0.0};
 #pragma unroll
-  for (int e = 0; e < 4; ++e) y[e * S + t] = apply_epilogue(A, acc[e], p[e], s, b, ld, e * S + t);
-  if (A.copy) {
+  for (int e = 0; e < 4; ++e) {
+    c128 v = apply_epilogue(A, acc[e], p[e], s, b, ld, e * S + t);
+    if (A.dots && valid) dot = cfmaconj(cscale(p[e], s), v, dot);
+    if (A.prev) {
+      const c128 q = A.prev[b * A.prev_ld + e * S + t];
+      v.re = fma(-bprev, q.re, v.re);
+      v.im = fma(-bprev, q.im, v.im);
+    }
+    if (valid) y[e * S + t] = v;
+  }
+  if (A.copy && valid) {
     c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
     for (int e = 0; e < 4; ++e) z[e * S + t] = cscale(p[e], s);
+  }
+  if (A.dots) {
+    __shared__ c128 red[256 / kWave];
+    block_dot_store(dot, red, A.dots + b * A.dots_ld + blockIdx.x);
   }
 }
 
@@ -396,8 +412,9 @@ __global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
     if (!c.active || A.j >= c.target) return;
     s = c.scale;
   }
-  const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const bool valid = i0 < n;
+  const long long i = valid ? i0 : 0;   // idle threads recompute element 0 and store nothing
   const c128* x = in + b * ld;
   c128 acc{0.0, 0.0};
   for (int t = 0; t < nterms; ++t) {
@@ -413,8 +430,23 @@ __global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
     const long long* cc = cols + T.col_offset;
     for (int c = 0; c < T.D; ++c) acc = cfma(row[c], x[base + cc[c]], acc);
   }
-  out[b * ld + i] = cscale(acc, s);
-  if (A.copy) A.copy[b * A.copy_ld + i] = cscale(x[i], s);
+  c128 v = cscale(acc, s);
+  c128 dot{0.0, 0.0};
+  if (A.dots && valid) dot = cfmaconj(cscale(x[i], s), v, dot);
+  if (A.prev) {
+    const double bprev = A.ctl[b].beta_prev;
+    const c128 q = A.prev[b * A.prev_ld + i];
+    v.re = fma(-bprev, q.re, v.re);
+    v.im = fma(-bprev, q.im, v.im);
+  }
+  if (valid) {
+    out[b * ld + i] = v;
+    if (A.copy) A.copy[b * A.copy_ld + i] = cscale(x[i], s);
+  }
+  if (A.dots) {
+    __shared__ c128 red[256 / kWave];
+    block_dot_store(dot, red, A.dots + b * A.dots_ld + blockIdx.x);
+  }
 }
 
 static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes, const int* taxes,
@@ -494,9 +526,20 @@ static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes
   return RPM_OK;
 }
 
+int apply_units(const rpm_operator* op) {
+  if (op->kind == 1) {
+    const PairOp* P = static_cast<const PairOp*>(op->impl);
+    if (P->use_tile && P->kernel != 1) return P->tile.ngroups;
+    return ceil_div(P->t.NLR, 256);
+  }
+  return ceil_div(static_cast<const LocalOp*>(op->impl)->n, 256);
+}
+
 int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int batch, long long ld,
                       const ApplyCtl& A, hipStream_t stream) {
   if (!op) return fail(RPM_E_INVALID, "NULL operator");
+  if ((A.prev || A.dots) && (!A.ctl || A.affine)) return fail(RPM_E_INVALID, "three-term options need a basis ctl");
+  if (A.dots && A.dots_ld < apply_units(op)) return fail(RPM_E_INVALID, "dots stride below the unit count");
   if (batch < 1 || batch > 65535) return fail(RPM_E_INVALID, "batch %d out of range", batch);
   if (ld < op->n) return fail(RPM_E_INVALID, "ld %lld smaller than the dimension %lld", ld, op->n);
   if (!in || !out) return fail(RPM_E_INVALID, "NULL vector");

@@ -169,15 +169,27 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
   }
 
   c128* __restrict__ y = out + b * ld;
+  const double bprev = A.prev ? A.ctl[b].beta_prev : 0.0;
+  c128 dot{0.0, 0.0};
 #pragma unroll
   for (int l = 0; l < LG; ++l)
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const long long idx = e * S + static_cast<long long>(l0 + l) * NR + r;
-      const c128 v = apply_epilogue(A, acc[l][e], psi[l][e], s, b, ld, idx);
+      c128 v = apply_epilogue(A, acc[l][e], psi[l][e], s, b, ld, idx);
+      if (A.dots) dot = cfmaconj(cscale(psi[l][e], s), v, dot);
+      if (A.prev) {
+        const c128 p = A.prev[b * A.prev_ld + idx];
+        v.re = fma(-bprev, p.re, v.re);
+        v.im = fma(-bprev, p.im, v.im);
+      }
       __builtin_nontemporal_store(v.re, &y[idx].re);
       __builtin_nontemporal_store(v.im, &y[idx].im);
     }
+  if (A.dots) {
+    __shared__ c128 red[NR / kWave];
+    block_dot_store(dot, red, A.dots + b * A.dots_ld + grp);
+  }
   if (A.copy) {
     c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
@@ -273,9 +285,17 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   }
 
   c128* __restrict__ y = out + b * ld;
+  const double bprev = A.prev ? A.ctl[b].beta_prev : 0.0;
+  c128 dot{0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    const c128 v = apply_epilogue(A, acc[e], psi[e], s, b, ld, e * S + own);
+    c128 v = apply_epilogue(A, acc[e], psi[e], s, b, ld, e * S + own);
+    if (A.dots) dot = cfmaconj(cscale(psi[e], s), v, dot);
+    if (A.prev) {
+      const c128 p = A.prev[b * A.prev_ld + e * S + own];
+      v.re = fma(-bprev, p.re, v.re);
+      v.im = fma(-bprev, p.im, v.im);
+    }
     __builtin_nontemporal_store(v.re, &y[e * S + own].re);
     __builtin_nontemporal_store(v.im, &y[e * S + own].im);
   }
@@ -283,6 +303,10 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     c128* z = A.copy + b * A.copy_ld;
 #pragma unroll
     for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(psi[e], s);
+  }
+  if (A.dots) {
+    __shared__ c128 red[NT / kWave];
+    block_dot_store(dot, red, A.dots + b * A.dots_ld + grp);
   }
 }
 

@@ -120,7 +120,7 @@ struct BasisCtl {
   int target;       // requested basis size
   int pad;
   double scale;     // 1/norm of the vector the next matvec normalises
-  double pad2;
+  double beta_prev; // beta_{j-1} of the Lanczos three-term step (0 for j = 0)
 };
 
 // Options of one batched H·psi launch (all nullable / off by default):
@@ -137,6 +137,13 @@ struct ApplyCtl {
   c128* copy = nullptr;
   long long copy_ld = 0;
   const c128* pairs = nullptr;
+  // Lanczos three-term step (with ctl): out -= ctl[b].beta_prev * prev[b * prev_ld + i], and
+  // per-workgroup partial sums of conj(s in) (s H in) -- alpha_j before the subtraction -- at
+  // dots[b * dots_ld + unit] (unit = tile group or block, apply_units() per vector).
+  const c128* prev = nullptr;
+  long long prev_ld = 0;
+  c128* dots = nullptr;
+  int dots_ld = 0;
   // affine epilogue (pair operators; Chebyshev / Clenshaw recurrences):
   //   out = aff_s * (H in) + aff_in * in + aff_c0 * x0 + aff_c1 * x1
   // with x0, x1 (nullable) per vector at b * ld like in and out.
@@ -182,4 +189,13 @@ __device__ inline void load_pair_coef(const c128* __restrict__ P, PairCoef& C) {
 
 int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int batch, long long ld,
                       const ApplyCtl& A, hipStream_t stream);
+// Number of partial sums per vector that operator_apply_ex writes to ApplyCtl::dots.
+int apply_units(const rpm_operator* op);
+
+// Block-wide sum of one complex value from every thread into dots[slot] (thread 0 writes).
+// Every thread of the block must call it.  smem: blockDim.x / kWave entries.
+__device__ inline void block_dot_store(c128 v, c128* smem, c128* dst) {
+  const c128 t = block_sum(v, smem);
+  if (threadIdx.x == 0) *dst = t;
+}
 }  // namespace rpm
