@@ -160,16 +160,27 @@ __global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dot
   if (threadIdx.x == 0) alpha[b * astride + j] = s.re;
 }
 
-// w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2.
+// w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2; with
+// alpha, the last column's coefficient is h_{k-1} + alpha_j (the three-term
+// step's alpha_j V_j subtracted together with the projection).
 constexpr int kUpdChunk = kThreads;
 __global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V, long long vstride, long long n, int k,
-                                                     const c128* __restrict__ h, int hstride, c128* __restrict__ W,
-                                                     double* __restrict__ norm_partial, int j, const BasisCtl* ctl) {
+                                                     const c128* __restrict__ h, int hstride,
+                                                     const double* __restrict__ alpha, int astride,
+                                                     c128* __restrict__ W, double* __restrict__ norm_partial, int j,
+                                                     const BasisCtl* ctl) {
   const int b = blockIdx.y;
   if (!basis_live(ctl, b, j, 1)) return;
   __shared__ double sm[kThreads / kWave];
+  __shared__ c128 hs[kMaxBasis];
   const c128* Vb = V + b * vstride;
-  const c128* hb = h + b * hstride;
+  if (threadIdx.x < k) {
+    c128 v = h[b * hstride + threadIdx.x];
+    if (alpha && threadIdx.x == k - 1) v.re += alpha[b * astride + j];
+    hs[threadIdx.x] = v;
+  }
+  __syncthreads();
+  const c128* hb = hs;
   c128* w = W + b * n;
   const long long i = static_cast<long long>(blockIdx.x) * kUpdChunk + threadIdx.x;
   c128 wv = i < n ? w[i] : cmake(0.0, 0.0);
@@ -191,61 +202,55 @@ __global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V,
   if (threadIdx.x == 0) norm_partial[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
 }
 
-// Fused pass over an LDS-staged chunk of the basis: finishes the three-term
-// step w <- w - alpha_j V_j (column j = k-1 of the chunk), then
-// partial[c][blk] = sum_x conj(V_c[x]) w[x] for all c < k.  Each block walks
-// kGsPer chunks of kGsChunk elements; V is read from HBM once per pass.
-// k <= kMaxBasis.
-constexpr int kGsChunk = 32;
-constexpr int kGsPer = 4;
-__global__ void __launch_bounds__(256) k_gs_pass(const c128* __restrict__ V, long long vstride, long long n, int k,
-                                                 const double* __restrict__ alpha, int astride, c128* __restrict__ W,
-                                                 c128* __restrict__ partial, long long pstride, int j, int need,
-                                                 const BasisCtl* ctl) {
+// Fused pass over the basis: partial[c][blk] = sum_x conj(V_c[x]) w'[x] for all
+// c < k with w' = w - alpha_j V_j (j = k-1), the finished three-term step.  w'
+// is formed in registers only (every wave reads w); k_update subtracts
+// alpha_j V_j together with the projection.
+// Wave q of the 4-wave block owns columns c = q, q+4, ..., one element per lane
+// per iteration, so every lane keeps up to 16 column loads in flight and no LDS
+// staging is needed; each block covers `per` consecutive 64-element runs and
+// ends with one wave reduction per column.  k <= kMaxBasis.
+constexpr int kDotWaves = 4;
+constexpr int kDotCols = kMaxBasis / kDotWaves;
+__global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __restrict__ V, long long vstride, long long n,
+                                                               int k, int per, const double* __restrict__ alpha,
+                                                               int astride, const c128* __restrict__ W,
+                                                               c128* __restrict__ partial, long long pstride, int j,
+                                                               int need, const BasisCtl* ctl) {
   const int b = blockIdx.y;
   if (!basis_live(ctl, b, j, need)) return;
-  __shared__ c128 vs[kMaxBasis][kGsChunk + 1];
-  __shared__ c128 ws[kGsChunk];
-  __shared__ c128 red[256];
   const c128* Vb = V + b * vstride;
-  c128* w = W + b * n;
-  const int tid = threadIdx.x;
-  const int nblk = gridDim.x;
+  const c128* w = W + b * n;
+  const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
   const double a = alpha ? alpha[b * astride + j] : 0.0;
-  const int dc = tid % 64, dq = tid / 64;   // dot mapping: column dc, 8-element part dq
-  c128 dot{0.0, 0.0};
-  for (int it = 0; it < kGsPer; ++it) {
-    const long long x0 = (static_cast<long long>(blockIdx.x) * kGsPer + it) * kGsChunk;
-    if (x0 >= n) break;
-    __syncthreads();
-    for (int s = tid; s < k * kGsChunk; s += 256) {
-      const int c = s / kGsChunk, i = s % kGsChunk;
-      const long long x = x0 + i;
-      vs[c][i] = x < n ? Vb[static_cast<long long>(c) * n + x] : cmake(0.0, 0.0);
-    }
-    if (tid < kGsChunk) {
-      const long long x = x0 + tid;
-      c128 wv = x < n ? w[x] : cmake(0.0, 0.0);
-      if (alpha) {
-        const c128 v = x < n ? Vb[static_cast<long long>(k - 1) * n + x] : cmake(0.0, 0.0);
-        wv.re = fma(-a, v.re, wv.re);
-        wv.im = fma(-a, v.im, wv.im);
-        if (x < n) w[x] = wv;
-      }
-      ws[tid] = wv;
-    }
-    __syncthreads();
-    if (dc < k) {
+  const c128* vj = Vb + static_cast<long long>(k - 1) * n;
+  c128 acc[kDotCols];
 #pragma unroll
-      for (int i = dq * 8; i < dq * 8 + 8; ++i) dot = cfmaconj(vs[dc][i], ws[i], dot);
+  for (int i = 0; i < kDotCols; ++i) acc[i] = cmake(0.0, 0.0);
+  const long long x0 = static_cast<long long>(blockIdx.x) * per * kWave + lane;
+  for (int it = 0; it < per; ++it) {
+    const long long x = x0 + static_cast<long long>(it) * kWave;
+    if (x >= n) break;
+    c128 wv = w[x];
+    if (alpha) {
+      const c128 v = vj[x];
+      wv.re = fma(-a, v.re, wv.re);
+      wv.im = fma(-a, v.im, wv.im);
+    }
+#pragma unroll
+    for (int i = 0; i < kDotCols; ++i) {
+      const int c = q + kDotWaves * i;
+      if (c < k) acc[i] = cfmaconj(Vb[static_cast<long long>(c) * n + x], wv, acc[i]);
     }
   }
-  __syncthreads();
-  red[tid] = dot;
-  __syncthreads();
-  if (tid < 64 && tid < k) {
-    const c128 t = cadd(cadd(red[tid], red[tid + 64]), cadd(red[tid + 128], red[tid + 192]));
-    partial[b * pstride + static_cast<long long>(tid) * nblk + blockIdx.x] = t;
+  const int nblk = gridDim.x;
+#pragma unroll
+  for (int i = 0; i < kDotCols; ++i) {
+    const int c = q + kDotWaves * i;
+    if (c < k) {
+      const c128 t = wave_sum(acc[i]);
+      if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
+    }
   }
 }
 
@@ -438,7 +443,7 @@ struct BasisWs {
 
 BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   const int nchunk = ceil_div(n, kChunk);
-  const int gsb = ceil_div(n, kGsChunk * kGsPer);
+  const int gsb = ceil_div(n, 8 * kWave);   // dots blocks at the smallest block size
   const int nupd = ceil_div(n, kUpdChunk);
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -479,6 +484,11 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   const int nchunk = ceil_div(n, kChunk);
   const int nupd = ceil_div(n, kUpdChunk);
   const int count = static_cast<int>(max_target < n ? max_target : n);
+  // elements per dots block: 8 x 64 (enough blocks to fill the chip for one vector); the same in
+  // batches, so a batched basis is bitwise the single-trajectory basis
+  const int dot_per = 8;
+  const int dot_blocks = ceil_div(n, static_cast<long long>(dot_per) * kWave);
+  if (dot_blocks > w.nblk) return fail(RPM_E_INVALID, "internal: dots partial buffer too small");
   hipLaunchKernelGGL(k_sumsq, dim3(nchunk, batch), dim3(kThreads), 0, s, states, n, sld, w.sq, d_targets);
   RPM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, d_targets, count, n);
@@ -505,15 +515,15 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     if (j == count - 1) break;
     const int k = j + 1;
     // pass 1: w -= alpha_j V[j], h = V^H w
-    hipLaunchKernelGGL(k_gs_pass, dim3(w.nblk, batch), dim3(256), 0, s, V, vstride, n, k, alpha, m, out, w.partial,
-                       w.pstride, j, 1, w.ctl);
+    hipLaunchKernelGGL(k_gs_dots, dim3(dot_blocks, batch), dim3(kDotWaves * kWave), 0, s, V, vstride, n, k, dot_per,
+                       alpha, m, out, w.partial, w.pstride, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, w.nblk, w.pstride, w.h, m,
+    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, dot_blocks, w.pstride, w.h, m,
                        nullptr, m, 0, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
     // pass 2: w -= V h, ||w||
-    hipLaunchKernelGGL(k_update, dim3(nupd, batch), dim3(kThreads), 0, s, V, vstride, n, k, w.h, m, out, w.npart, j,
-                       w.ctl);
+    hipLaunchKernelGGL(k_update, dim3(nupd, batch), dim3(kThreads), 0, s, V, vstride, n, k, w.h, m, alpha, m, out,
+                       w.npart, j, w.ctl);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_finish_beta, dim3(batch), dim3(kThreads), 0, s, w.npart, nupd, beta, m, j, tol, w.ctl);
     RPM_LAUNCH_CHECK();
