@@ -136,7 +136,12 @@ def test_production_hamiltonian_uses_pair_stencil_and_matches_oracle_terms():
     assert [t.label for t in h.terms] == [t.label for t in o.terms]
     for a, b in zip(h.terms, o.terms):
         assert a.axes == b.axes
-        np.testing.assert_array_equal(a.operator, b.operator)
+        if "quadrupole" in a.label:
+            # the product uses the published tensors; the oracle rebuilds them with the local
+            # LAPACK, which may differ in the last bit
+            np.testing.assert_allclose(a.operator, b.operator, rtol=0, atol=1e-13)
+        else:
+            np.testing.assert_array_equal(a.operator, b.operator)
     form = engine._pair_form(h.dimensions, h.terms, h.left_axes, h.right_axes)
     ldims, rdims, blocks, pair = form
     assert ldims == [3, 3, 2, 2, 2, 2, 2] and rdims == [3, 2, 2, 2, 2, 2, 2]
