@@ -5,8 +5,14 @@ production operator (theta = 90 deg, 50 uT, Efimova-Hore D/J = -3, 14N
 quadrupoles on).  value = H·psi per second over the timed steps; the roofline
 object prices the kernel's algorithmic bytes (read psi + write H psi,
 32 B per amplitude = 7,077,888 B per vector) against the 8 TB/s HBM peak.
+The same JSON line carries the other two parts of the metric from short
+probes: "batched" (32 trajectories = 8 field orientations x 4 random-phase
+vectors through the reference's adaptive Lanczos rules, against the plain
+one-trajectory-at-a-time path) and "gradient" (yield plus adjoint gradient
+over yield alone).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
+    python bench.py --mode propagate|batch|grad|copy ...
 """
 
 from __future__ import annotations
@@ -87,7 +93,7 @@ def propagate_probe(h, args) -> int:
     return 0
 
 
-def batch_probe(h, args) -> int:
+def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
     """Batched trajectories (random-phase vectors x field orientations) against the plain per-vector path.
 
     Both run the same trajectories over the same short horizon with the
@@ -98,8 +104,9 @@ def batch_probe(h, args) -> int:
     import torch
     from rpm_amd import batch, core
     cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
-    times = np.arange(int(round(args.horizon / 0.00125)) + 1) * 0.00125
-    B = args.batch
+    horizon_req = args.horizon if horizon is None else horizon
+    times = np.arange(int(round(horizon_req / 0.00125)) + 1) * 0.00125
+    B = args.batch if B is None else B
     orients = [(float(t), 0.0, 0.05) for t in np.linspace(0.0, 180.0, max(1, B // 4))]
     hs, states = [], []
     for i in range(B):
@@ -116,7 +123,7 @@ def batch_probe(h, args) -> int:
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     print(f"[bench] batched {B} trajectories: {t_batch:.3f} s, {bp.rounds} rounds", file=sys.stderr, flush=True)
-    nplain = min(B, args.plain_sample)
+    nplain = min(B, args.plain_sample if plain_sample is None else plain_sample)
     t_plain = 0.0
     for i in range(nplain):
         prop = core.make_propagator(hs[i], cfg, probe_horizon=float(times[-1]))
@@ -134,22 +141,23 @@ def batch_probe(h, args) -> int:
     batched_rate = B * horizon / t_batch
     plain_rate = nplain * horizon / t_plain if nplain else float("nan")
     matvecs = sum(r.stats.matvecs for r in res)
-    print(json.dumps({"mode": "batch", "trajectories": B, "horizon_us": horizon, "seconds_batched": t_batch,
-                      "trajectory_us_per_s_batched": batched_rate, "plain_sample": nplain,
-                      "trajectory_us_per_s_plain": plain_rate, "speedup_vs_plain": batched_rate / plain_rate,
-                      "yields_per_s_batched_equiv": batched_rate / cfg.maximum_time_us,
-                      "matvecs": matvecs, "rounds": bp.rounds}), flush=True)
-    return 0
+    return {"mode": "batch", "trajectories": B, "orientations": len(orients), "horizon_us": horizon,
+            "seconds_batched": t_batch, "trajectory_us_per_s_batched": batched_rate, "plain_sample": nplain,
+            "trajectory_us_per_s_plain": plain_rate, "speedup_vs_plain": batched_rate / plain_rate,
+            "yields_per_s_batched_equiv": batched_rate / cfg.maximum_time_us,
+            "yields_per_s_plain_equiv": plain_rate / cfg.maximum_time_us,
+            "full_horizon_us": cfg.maximum_time_us, "matvecs": matvecs, "rounds": bp.rounds}
 
 
-def grad_probe(h, args) -> int:
+def grad_probe(h, args, B=None, horizon=None) -> dict:
     """Yield-only vs yield + adjoint gradient for a batch of production trajectories over a short horizon."""
     import numpy as np
     import torch
     from rpm_amd import adjoint, batch, core
     cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
-    times = np.arange(int(round(args.horizon / 0.00125)) + 1) * 0.00125
-    B = args.batch
+    horizon_req = args.horizon if horizon is None else horizon
+    times = np.arange(int(round(horizon_req / 0.00125)) + 1) * 0.00125
+    B = args.batch if B is None else B
     thetas = list(np.linspace(10.0, 170.0, B))
     phis = [0.0] * B
     fields = [50.0] * B
@@ -169,11 +177,9 @@ def grad_probe(h, args) -> int:
     fgrad, _, rec = prob.backward()
     torch.cuda.synchronize()
     t_grad = time.perf_counter() - t0
-    print(json.dumps({"mode": "grad", "trajectories": B, "horizon_us": float(times[-1]), "intervals": times.size - 1,
-                      "seconds_yield": t_yield, "seconds_adjoint": t_grad,
-                      "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield,
-                      "psi0_recovery_error": rec, "grad_theta_first": float(fgrad[0, 0])}), flush=True)
-    return 0
+    return {"mode": "grad", "trajectories": B, "horizon_us": float(times[-1]), "intervals": times.size - 1,
+            "parameters": ["theta", "phi", "|B|"], "seconds_yield": t_yield, "seconds_adjoint": t_grad,
+            "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield, "psi0_recovery_error": rec}
 
 
 def main() -> int:
@@ -183,6 +189,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched-yield and gradient probes")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
@@ -208,9 +215,11 @@ def main() -> int:
     if args.mode == "propagate":
         return propagate_probe(h, args)
     if args.mode == "batch":
-        return batch_probe(h, args)
+        print(json.dumps(batch_probe(h, args)), flush=True)
+        return 0
     if args.mode == "grad":
-        return grad_probe(h, args)
+        print(json.dumps(grad_probe(h, args)), flush=True)
+        return 0
     h._device_op().set_kernel(args.kernel)
     n = h.dimension
     B = args.batch
@@ -270,6 +279,15 @@ def main() -> int:
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
+    if not args.no_extras:
+        # the other two parts of the metric, from short probes of the same production operator
+        h._device_op().set_kernel(0)
+        del x, y
+        torch.cuda.empty_cache()
+        print("[bench] batched-trajectory probe", file=sys.stderr, flush=True)
+        result["batched"] = batch_probe(h, args, B=32, horizon=0.25, plain_sample=4)
+        print("[bench] gradient probe", file=sys.stderr, flush=True)
+        result["gradient"] = grad_probe(h, args, B=8, horizon=0.05)
     print(json.dumps(result), flush=True)
     return 0
 

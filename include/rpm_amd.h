@@ -37,7 +37,6 @@
  *   rpm_axis_apply             local_unitary_apply                     R/poster_reproduction/core.py:285-292
  *   rpm_operator_apply_affine  Chebyshev/Clenshaw step of the adjoint sweep (no reference counterpart: the
  *                              reference has no gradients; dH/dp is its Zeeman form R/poster_reproduction/core.py:186-194)
- *   rpm_singlet_kick           costate source 2 W_k P psi_k of the Simpson yield R/rpm_quantum_needle_jd_sweeps.py:418-420
  *   rpm_cross_density          electron 4x4 of <lambda| dH/dp |psi> for electron-only dH/dp (no reference counterpart)
  */
 #ifndef RPM_AMD_H
@@ -192,35 +191,33 @@ size_t rpm_cross_density_workspace_bytes(long long nn, int batch);
 int rpm_cross_density(const void* d_a, const void* d_b, int batch, long long ld, long long nn, double weight,
                       void* d_acc, void* d_workspace, size_t workspace_bytes, void* stream);
 
-/* d_g[b] += coef * P d_psi[b] with P the singlet projector |S><S| (x) 1 (N = 4 nn). */
-int rpm_singlet_kick(const void* d_psi, void* d_g, int batch, long long ld, long long nn, double coef,
-                     void* stream);
-
 /* d_out[b] = (1 (x) U (x) 1) d_in[b]: U (d x d complex, row-major) on tensor axis `axis` of the
  * C-order state with dimensions dims[naxes]; d = dims[axis] <= 16. */
 int rpm_axis_apply(int naxes, const int* dims, int axis, const void* d_U, const void* d_in, void* d_out,
                    int batch, long long ld, void* stream);
 
 /*
- * Adjoint sweep of the Simpson singlet yield Phi = sum_k W_k <psi_k|P|psi_k> on the output grid
- * t_0..t_M (no reference counterpart; the reference computes Phi only,
- * R/rpm_quantum_needle_jd_sweeps.py:418-420).  Runs backward from d_psi = psi_M (overwritten
- * with psi_0): per interval [t_{k-1}, t_k] the costate gets g += 2 W_k P psi_k, then psi and g
- * are carried back by the Chebyshev expansion of exp(+i H h) (X = (H - shift)/half_width,
- * coefficients c_n = exp(-i shift h) a_n(half_width h), Clenshaw recurrences), and
- *   d_acc[slot[k-1]][b] += sum_n w_n / half_width * C(nu_{n+1}, T_n(X) psi_{k-1})
- * (w_0 = 1, w_n = 2, C as in rpm_cross_density, nu the Clenshaw vectors of g), so that
- * dPhi/dp = Re sum_{e,f} (dH_b/dp)[e][f] acc[e][f] for every electron-only dH/dp.
- * h_int[M] interval lengths (host), weights[M+1] (host), cls[M] interval class, nterms[ncls]
- * series lengths K (2..kmax), coef[ncls][kmax+1] complex c_n (host, interleaved), seg[M]
- * electron-matrix segment (d_pairs: nseg x batch x 16 complex, nullable), slot[M] accumulator
- * slot (d_acc: nslot x batch x 16 complex, accumulated into).
+ * Adjoint sweep of the Simpson singlet yield Phi = sum_k W_k <psi_k|P|psi_k> on an output grid
+ * (no reference counterpart; the reference computes Phi only, R/rpm_quantum_needle_jd_sweeps.py:418-420).
+ * The grid is cut into nmacro steps [t_a, t_b]; step k has class cls[k] (series length
+ * nterms[c] = K <= kmax, coefficients coef[c][0..K] (host, interleaved complex) of
+ * exp(-i H (t_b - t_a)) = sum_n c_n T_n(X), X = (H - shift)/half_width, and the (kmax+1)^2 complex
+ * matrix d_G[c] (device) with G[m][j] = sum_{kicks t in (t_a, t_b]} 2 w(t) conj(c_m(t - t_a)) c_j(t - t_a)),
+ * a weight multiplier mult[k] (W = mult[k] w), electron-matrix segment seg[k] (d_pairs: nseg x batch
+ * x 16 complex, nullable) and accumulator slot slot[k].  Running backward from d_psi = psi(t_end)
+ * (overwritten with psi(0)), each step carries psi back by Clenshaw's recurrence, forms
+ * u_j = T_j(X) psi(t_a), and runs Clenshaw over the costate with the vector seeds
+ * z_m = conj(c_m) g + mult P sum_j G[m][j] u_j, accumulating
+ *   d_acc[slot][b] += sum_n w_n / half_width * C(nu_{n+1}, u_n)   (w_0 = 1, w_n = 2,
+ * C as in rpm_cross_density), so that dPhi/dp = Re sum_{e,f} (dH_b/dp)[e][f] acc[e][f] for every
+ * electron-only dH/dp.
  */
 size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch, int kmax);
-int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nint, const double* h_int,
-                      const double* weights, double shift, double half_width, int ncls, const int* nterms,
-                      int kmax, const double* coef, const int* cls, const void* d_pairs, const int* seg,
-                      void* d_acc, const int* slot, void* d_workspace, size_t workspace_bytes, void* stream);
+int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro, double shift,
+                      double half_width, int ncls, const int* nterms, int kmax, const double* coef,
+                      const void* d_G, const int* cls, const double* mult, const void* d_pairs,
+                      const int* seg, void* d_acc, const int* slot, void* d_workspace, size_t workspace_bytes,
+                      void* stream);
 
 /*
  * Embedded error estimates for a ladder of candidate dimensions.

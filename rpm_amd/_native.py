@@ -69,13 +69,12 @@ SIGNATURES = {
                                                  ctypes.c_double, _vp]),
     "rpm_cross_density_workspace_bytes": (_sz, [_ll, ctypes.c_int]),
     "rpm_cross_density": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _ll, _ll, ctypes.c_double, _vp, _vp, _sz, _vp]),
-    "rpm_singlet_kick": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _ll, _ll, ctypes.c_double, _vp]),
     "rpm_axis_apply": (ctypes.c_int, [ctypes.c_int, _c_int_p, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _ll,
                                       _vp]),
     "rpm_adjoint_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
-    "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, _c_double_p, _c_double_p,
-                                         ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_int_p, ctypes.c_int,
-                                         _c_double_p, _c_int_p, _vp, _c_int_p, _vp, _c_int_p, _vp, _sz, _vp]),
+    "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,
+                                         _c_double_p, _vp, _c_int_p, _vp, _c_int_p, _vp, _sz, _vp]),
 }
 
 _lib = None

@@ -116,36 +116,82 @@ def spectral_bounds(hamiltonian, pairs=None):
     return lo + float(ev.min()), hi + float(ev.max())
 
 
-def _interval_classes(h: np.ndarray, rtol: float = 1e-12):
-    """Group interval lengths equal to rtol; returns (class of each interval, representative lengths)."""
-    reps: list[float] = []
-    cls = np.empty(h.size, dtype=np.int32)
-    for i, x in enumerate(h):
-        for c, r in enumerate(reps):
-            if abs(x - r) <= rtol * r:
-                cls[i] = c
-                break
-        else:
-            reps.append(float(x))
-            cls[i] = len(reps) - 1
-    return cls, np.array(reps)
+MACRO_INTERVALS = 16      # output-grid intervals per backward Chebyshev step
+
+
+def chebyshev_series_fixed(h: float, shift: float, half_width: float, K: int) -> np.ndarray:
+    """c_0..c_K of exp(-i H h) in the Chebyshev basis of X = (H - shift)/half_width (fixed length K)."""
+    J = _bessel_j(K, half_width * h)
+    n = np.arange(K + 1)
+    a = 2.0 * ((-1j) ** n) * J
+    a[0] = J[0]
+    return np.exp(-1j * shift * h) * a
+
+
+def _macro_steps(M: int, seg: np.ndarray, slot: np.ndarray, per: int):
+    """Cut M grid intervals into steps (a, b) of at most ``per`` intervals, each within one segment and slot."""
+    steps = []
+    a = 0
+    while a < M:
+        b = a + 1
+        while b < M and b - a < per and seg[b] == seg[a] and slot[b] == slot[a]:
+            b += 1
+        steps.append((a, b))
+        a = b
+    return steps
+
+
+@dataclass
+class _StepClass:
+    K: int
+    coef: np.ndarray      # (K+1,) complex: c_n of the whole step
+    G: np.ndarray         # (K+1, K+1) complex kick matrix
+
+
+def _step_classes(grid, weights, steps, shift, half, decay_time):
+    """Series and kick matrices per step, shared between steps that differ only by the weight factor.
+
+    W_k = mult * w_k with mult = exp(-t_a / decay_time) (1 without a decay time), and
+    G[m][j] = sum_k 2 w_k conj(c_m(t_k - t_a)) c_j(t_k - t_a) over the output times of the step.
+    """
+    classes, index, cls, mult = [], {}, [], []
+    for a, b in steps:
+        tk = grid[a + 1:b + 1] - grid[a]
+        m = math.exp(-grid[a] / decay_time) if decay_time else 1.0
+        wk = weights[a + 1:b + 1] / m
+        key = (round(float(tk[-1]), 15), tuple(np.round(tk, 15)), tuple(np.round(wk, 14)))
+        c = index.get(key)
+        if c is None:
+            K, cdelta = chebyshev_series(float(tk[-1]), shift, half)
+            ck = np.stack([chebyshev_series_fixed(float(t), shift, half, K) for t in tk])
+            G = np.einsum("k,km,kj->mj", 2.0 * wk, ck.conj(), ck)
+            c = len(classes)
+            classes.append(_StepClass(K, cdelta, G))
+            index[key] = c
+        cls.append(c)
+        mult.append(m)
+    return classes, np.asarray(cls, dtype=np.int32), np.asarray(mult, dtype=float)
 
 
 @dataclass
 class AdjointResult:
     densities: np.ndarray        # (nslot, B, 4, 4) complex: accumulated electron cross-densities
     initial_states: object       # device (B, N): psi_0 recovered by the backward sweep
-    chebyshev_terms: tuple       # K per interval class
+    chebyshev_terms: tuple       # K per step class
     bounds: tuple                # spectral enclosure used
+    steps: int                   # backward Chebyshev steps
 
 
 def adjoint_densities(hamiltonian, final_states, times_us, weights, *, pairs=None, segment_of_interval=None,
-                      slot_of_interval=None, nslot: int = 1) -> AdjointResult:
+                      slot_of_interval=None, nslot: int = 1, decay_time=None,
+                      intervals_per_step: int = MACRO_INTERVALS) -> AdjointResult:
     """Run the backward sweep from the final states of a forward run.
 
     ``times_us``: sweep grid t_0 = 0 < ... < t_M (the output grid, plus any
     segment boundaries); ``weights``: W_k of Phi = sum_k W_k <psi_k|P|psi_k>
-    on that grid (0 at points that are not output times).
+    on that grid (0 at points that are not output times; W_0 is not used,
+    psi_0 does not depend on the parameters); ``decay_time``: tau when
+    W_k = s_k exp(-t_k/tau)/tau (lets equal steps share their tables).
     ``final_states``: complex128 GPU tensor (B, N) with psi(t_M) of each
     trajectory (left untouched; a copy is swept).  ``pairs``: electron
     matrices, (B, 4, 4) for one field per trajectory or (S, B, 4, 4) per
@@ -159,60 +205,65 @@ def adjoint_densities(hamiltonian, final_states, times_us, weights, *, pairs=Non
     times = np.asarray(times_us, dtype=float)
     if times.ndim != 1 or times.size < 2 or times[0] != 0.0 or np.any(np.diff(times) <= 0.0):
         raise ValueError("The output grid must start at 0 and increase strictly.")
+    w = np.asarray(weights, dtype=float)
+    if w.shape != times.shape:
+        raise ValueError("weights must match the sweep grid")
     psi = final_states.to(torch.complex128).reshape(final_states.shape[0], -1).contiguous().clone()
     B, n = psi.shape
     if n != h.dimension:
         raise ValueError("State dimension does not match the Hamiltonian.")
     M = times.size - 1
-    hint = np.diff(times)
-    cls, reps = _interval_classes(hint)
+    p = None
     d_pairs = None
-    seg = None
+    seg = np.zeros(M, dtype=np.int32)
     if pairs is not None:
         p = np.asarray(pairs, dtype=complex)
         if p.ndim == 3:
             p = p[None]
         if p.shape[1:] != (B, 4, 4):
             raise ValueError(f"pairs must have shape (B, 4, 4) or (S, B, 4, 4) with B = {B}")
-        seg = np.zeros(M, dtype=np.int32) if segment_of_interval is None else np.asarray(segment_of_interval,
-                                                                                         dtype=np.int32)
+        if segment_of_interval is not None:
+            seg = np.asarray(segment_of_interval, dtype=np.int32)
         if seg.shape != (M,) or seg.min() < 0 or seg.max() >= p.shape[0]:
             raise ValueError("segment_of_interval must give a valid segment for each interval")
         d_pairs = torch.from_numpy(np.ascontiguousarray(p.reshape(p.shape[0], B, 16))).to(psi.device)
     slot = np.zeros(M, dtype=np.int32) if slot_of_interval is None else np.asarray(slot_of_interval, dtype=np.int32)
     if slot.shape != (M,) or slot.min() < 0 or slot.max() >= nslot:
         raise ValueError("slot_of_interval must give a slot in 0..nslot-1 for each interval")
-    lo, hi = spectral_bounds(h, p if pairs is not None else None)
+    lo, hi = spectral_bounds(h, p)
     shift = 0.5 * (lo + hi)
     half = 0.5 * (hi - lo) * (1.0 + 1e-9) + 1e-12
-    series = [chebyshev_series(float(x), shift, half) for x in reps]
-    kmax = max(k for k, _ in series)
-    coef = np.zeros((len(series), kmax + 1), dtype=complex)
-    nterms = np.zeros(len(series), dtype=np.int32)
-    for c, (k, cn) in enumerate(series):
-        nterms[c] = k
-        coef[c, :k + 1] = cn
-    w = np.asarray(weights, dtype=float)
-    if w.shape != times.shape:
-        raise ValueError("weights must match the sweep grid")
+    steps = _macro_steps(M, seg, slot, max(1, int(intervals_per_step)))
+    classes, cls, mult = _step_classes(times, w, steps, shift, half, decay_time)
+    kmax = max(c.K for c in classes)
+    if kmax > 127:
+        raise ValueError("sweep steps too long for the Chebyshev tables (use fewer intervals per step)")
+    coef = np.zeros((len(classes), kmax + 1), dtype=complex)
+    Gs = np.zeros((len(classes), kmax + 1, kmax + 1), dtype=complex)
+    nterms = np.zeros(len(classes), dtype=np.int32)
+    for c, sc in enumerate(classes):
+        nterms[c] = sc.K
+        coef[c, :sc.K + 1] = sc.coef
+        Gs[c, :sc.K + 1, :sc.K + 1] = sc.G
+    step_seg = np.ascontiguousarray(np.array([seg[a] for a, _ in steps], dtype=np.int32))
+    step_slot = np.ascontiguousarray(np.array([slot[a] for a, _ in steps], dtype=np.int32))
     acc = torch.zeros((nslot, B, 16), dtype=torch.complex128, device=psi.device)
+    dG = torch.from_numpy(np.ascontiguousarray(Gs)).to(psi.device)
     ws = nat.workspace(lib.rpm_adjoint_workspace_bytes(ctypes.c_void_p(op.handle), B, int(kmax)))
     coef_ri = np.ascontiguousarray(np.stack([coef.real, coef.imag], axis=-1).reshape(-1))
-    hint_c = np.ascontiguousarray(hint)
-    w_c = np.ascontiguousarray(w)
     cls_c = np.ascontiguousarray(cls)
-    slot_c = np.ascontiguousarray(slot)
+    mult_c = np.ascontiguousarray(mult)
     dp = ctypes.POINTER(ctypes.c_double)
     ip = ctypes.POINTER(ctypes.c_int)
     nat.check(lib.rpm_adjoint_sweep(
-        ctypes.c_void_p(op.handle), B, ctypes.c_void_p(nat.ptr(psi)), M, hint_c.ctypes.data_as(dp),
-        w_c.ctypes.data_as(dp), shift, half, len(series), nterms.ctypes.data_as(ip), int(kmax),
-        coef_ri.ctypes.data_as(dp), cls_c.ctypes.data_as(ip),
+        ctypes.c_void_p(op.handle), B, ctypes.c_void_p(nat.ptr(psi)), len(steps), shift, half, len(classes),
+        nterms.ctypes.data_as(ip), int(kmax), coef_ri.ctypes.data_as(dp), ctypes.c_void_p(nat.ptr(dG)),
+        cls_c.ctypes.data_as(ip), mult_c.ctypes.data_as(dp),
         ctypes.c_void_p(nat.ptr(d_pairs) if d_pairs is not None else None),
-        seg.ctypes.data_as(ip) if seg is not None else None, ctypes.c_void_p(nat.ptr(acc)),
-        slot_c.ctypes.data_as(ip), ctypes.c_void_p(nat.ptr(ws)), ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
+        step_seg.ctypes.data_as(ip) if d_pairs is not None else None, ctypes.c_void_p(nat.ptr(acc)),
+        step_slot.ctypes.data_as(ip), ctypes.c_void_p(nat.ptr(ws)), ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
     dens = acc.cpu().numpy().reshape(nslot, B, 4, 4)
-    return AdjointResult(dens, psi, tuple(int(k) for k, _ in series), (lo, hi))
+    return AdjointResult(dens, psi, tuple(int(c.K) for c in classes), (lo, hi), len(steps))
 
 
 def contract(dH: np.ndarray, densities: np.ndarray) -> np.ndarray:
@@ -332,7 +383,8 @@ class FieldYield:
             raise RuntimeError("backward() needs a forward() first")
         sv = self._saved
         adj = adjoint_densities(self.base, sv["final"], sv["grid"], sv["w"], pairs=sv["seg_pairs"],
-                                segment_of_interval=sv["seg_idx"], slot_of_interval=sv["seg_idx"], nslot=sv["nseg"])
+                                segment_of_interval=sv["seg_idx"], slot_of_interval=sv["seg_idx"], nslot=sv["nseg"],
+                                decay_time=self.tau)
         fgrad = contract(sv["dH"], adj.densities.sum(axis=0))
         pgrad = None
         if sv["pulse_slots"]:

@@ -1,7 +1,7 @@
 // Device pieces of the adjoint (costate) sweep and of segmented propagation:
 //   * electron cross-densities C[e][e'] = sum_n conj(a[e, n]) b[e', n], the
 //     4x4 reduced objects every electron-only derivative dH/dp needs;
-//   * the singlet kick g += c P psi of the costate at an output time;
+//   * the costate seeds of the Simpson-yield adjoint sweep;
 //   * an instantaneous local unitary on one tensor axis of a state
 //     (R/poster_reproduction/core.py:285-292).
 // All batched over vectors (grid.y); reductions are two-stage in a fixed order.
@@ -58,22 +58,6 @@ __global__ void k_density_finish(const c128* __restrict__ partial, int nblk, dou
   c128& dst = acc[b * 16 + i];
   dst.re = fma(weight, s.re, dst.re);
   dst.im = fma(weight, s.im, dst.im);
-}
-
-// g[e=1] += c (psi_1 - psi_2)/2, g[e=2] -= c (psi_1 - psi_2)/2   (P = |S><S|, S = (|01> - |10>)/sqrt2)
-__global__ void __launch_bounds__(256) k_singlet_kick(const c128* __restrict__ psi, c128* __restrict__ g, long long ld,
-                                                      long long nn, double c) {
-  const long long b = blockIdx.y;
-  const long long n = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  if (n >= nn) return;
-  const c128* x = psi + b * ld;
-  c128* y = g + b * ld;
-  const c128 p1 = x[nn + n], p2 = x[2 * nn + n];
-  const double re = 0.5 * c * (p1.re - p2.re), im = 0.5 * c * (p1.im - p2.im);
-  y[nn + n].re += re;
-  y[nn + n].im += im;
-  y[2 * nn + n].re -= re;
-  y[2 * nn + n].im -= im;
 }
 
 // out = (1 (x) U (x) 1) in on the axis with dimension d and C-order stride `inner`.
@@ -134,16 +118,6 @@ extern "C" int rpm_cross_density(const void* d_a, const void* d_b, int batch, lo
   return RPM_OK;
 }
 
-extern "C" int rpm_singlet_kick(const void* d_psi, void* d_g, int batch, long long ld, long long nn, double coef,
-                                void* stream) {
-  if (!d_psi || !d_g || d_psi == d_g || batch < 1 || batch > 65535 || nn < 1 || ld < 4 * nn)
-    return fail(RPM_E_INVALID, "rpm_singlet_kick: bad arguments");
-  hipLaunchKernelGGL(k_singlet_kick, dim3(ceil_div(nn, 256), batch), dim3(256), 0, static_cast<hipStream_t>(stream),
-                     static_cast<const c128*>(d_psi), static_cast<c128*>(d_g), ld, nn, coef);
-  RPM_LAUNCH_CHECK();
-  return RPM_OK;
-}
-
 extern "C" int rpm_axis_apply(int naxes, const int* dims, int axis, const void* d_U, const void* d_in, void* d_out,
                               int batch, long long ld, void* stream) {
   if (!dims || !d_U || !d_in || !d_out || d_in == d_out || naxes < 1 || naxes > 64 || axis < 0 || axis >= naxes ||
@@ -166,6 +140,43 @@ extern "C" int rpm_axis_apply(int naxes, const int* dims, int axis, const void* 
 }
 
 // ---------------------------------------------------------------- adjoint sweep
+
+namespace rpm {
+
+// chi[x] = u[e=1][x] - u[e=2][x] for x < nn (the singlet part of u, times sqrt2).
+__global__ void __launch_bounds__(256) k_singlet_diff(const c128* __restrict__ u, c128* __restrict__ chi, long long ld,
+                                                      long long cld, long long nn) {
+  const long long b = blockIdx.y;
+  const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (x >= nn) return;
+  const c128 p = u[b * ld + nn + x], q = u[b * ld + 2 * nn + x];
+  chi[b * cld + x] = csub(p, q);
+}
+
+// Costate seed of one Chebyshev order m:
+//   out = cg g + P sum_j G[j] u_j,  (P v)[1] = -(P v)[2] = (v[1] - v[2]) / 2,
+// with the u_j given through chi_j = u_j[1] - u_j[2] (chi: nj blocks of cstride).
+__global__ void __launch_bounds__(256) k_seed(const c128* __restrict__ g, c128 cg, const c128* __restrict__ chi,
+                                              long long cstride, long long cld, int nj, const c128* __restrict__ G,
+                                              double mult, c128* __restrict__ out, long long ld, long long nn) {
+  const long long b = blockIdx.y;
+  const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  __shared__ c128 gs[128];
+  for (int j = threadIdx.x; j < nj; j += 256) gs[j] = G[j];
+  __syncthreads();
+  if (x >= nn) return;
+  c128 s{0.0, 0.0};
+  for (int j = 0; j < nj; ++j) s = cfma(gs[j], chi[j * cstride + b * cld + x], s);
+  s = cscale(s, 0.5 * mult);
+  const c128* gb = g + b * ld;
+  c128* ob = out + b * ld;
+  ob[x] = cfma(cg, gb[x], cmake(0.0, 0.0));
+  ob[nn + x] = cfma(cg, gb[nn + x], s);
+  ob[2 * nn + x] = cfma(cg, gb[2 * nn + x], cmake(-s.re, -s.im));
+  ob[3 * nn + x] = cfma(cg, gb[3 * nn + x], cmake(0.0, 0.0));
+}
+
+}  // namespace rpm
 
 namespace {
 
@@ -206,29 +217,19 @@ int density(const SweepCtx& C, const c128* a, const c128* b, double weight, c128
 
 inline c128 conjc(const double* c) { return cmake(c[0], -c[1]); }
 
-// out = sum_n conj(c_n) T_n(X) seed by Clenshaw's recurrence (K >= 2 terms beyond c_0).
-// With u != NULL, also acc += sum_n w_n / hs * C(nu_{n+1}, u_n) (w_0 = 1, w_n = 2) for the
-// Clenshaw vectors nu.  ra, rb: scratch vectors.
-int clenshaw(const SweepCtx& C, const c128* seed, const double* coef, int K, const c128* pairs, c128* const* u,
-             c128* acc, c128* ra, c128* rb, c128* out) {
+// out = sum_n conj(c_n) T_n(X) seed by Clenshaw's recurrence (K >= 2).  ra, rb: scratch.
+int clenshaw_state(const SweepCtx& C, const c128* seed, const double* coef, int K, const c128* pairs, c128* ra,
+                   c128* rb, c128* out) {
   const double s2 = 2.0 / C.hs, a2 = -2.0 * C.shift / C.hs;
-  int rc = scaled_copy(C, conjc(coef + 2 * K), seed, ra);          // nu_K
+  int rc = scaled_copy(C, conjc(coef + 2 * K), seed, ra);
   if (rc) return rc;
-  if (u && (rc = density(C, ra, u[K - 1], (K - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
-  rc = affine(C, ra, rb, pairs, s2, a2, seed, conjc(coef + 2 * (K - 1)), nullptr, 0.0);   // nu_{K-1}
-  if (rc) return rc;
-  if (u && (rc = density(C, rb, u[K - 2], (K - 2 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
-  c128* hi = ra;   // nu_{n+2}
-  c128* lo = rb;   // nu_{n+1}
+  if ((rc = affine(C, ra, rb, pairs, s2, a2, seed, conjc(coef + 2 * (K - 1)), nullptr, 0.0))) return rc;
+  c128* hi = ra;
+  c128* lo = rb;
   for (int n = K - 2; n >= 1; --n) {
-    rc = affine(C, lo, hi, pairs, s2, a2, seed, conjc(coef + 2 * n), hi, -1.0);            // nu_n over nu_{n+2}
-    if (rc) return rc;
-    if (u && (rc = density(C, hi, u[n - 1], (n - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
-    c128* t = hi;
-    hi = lo;
-    lo = t;
+    if ((rc = affine(C, lo, hi, pairs, s2, a2, seed, conjc(coef + 2 * n), hi, -1.0))) return rc;
+    std::swap(hi, lo);
   }
-  // lo = nu_1, hi = nu_2
   return affine(C, lo, out, pairs, 1.0 / C.hs, -C.shift / C.hs, seed, conjc(coef), hi, -1.0);
 }
 
@@ -236,25 +237,27 @@ int clenshaw(const SweepCtx& C, const c128* seed, const double* coef, int K, con
 
 extern "C" size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch, int kmax) {
   if (!op || batch < 1 || kmax < 2) return 0;
-  const size_t vec = static_cast<size_t>(op->n) * batch * sizeof(c128);
-  const size_t vecs = 6 + static_cast<size_t>(kmax);   // psi', g', 2 ring, u_1..u_{K-1} (+1 spare)
-  return vecs * ((vec + 255) & ~size_t(255)) + rpm_cross_density_workspace_bytes(op->n / 4, batch) + 256;
+  const size_t vec = ((static_cast<size_t>(op->n) * batch * sizeof(c128)) + 255) & ~size_t(255);
+  const size_t qvec = ((static_cast<size_t>(op->n / 4) * batch * sizeof(c128)) + 255) & ~size_t(255);
+  // psi', g (2), ring a/b, seed z, u_1..u_K; then chi_0..chi_K
+  const size_t vecs = 6 + static_cast<size_t>(kmax);
+  return vecs * vec + (kmax + 1) * qvec + rpm_cross_density_workspace_bytes(op->n / 4, batch) + 256;
 }
 
-extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nint, const double* h_int,
-                                 const double* weights, double shift, double half_width, int ncls,
-                                 const int* nterms, int kmax, const double* coef, const int* cls,
-                                 const void* d_pairs, const int* seg, void* d_acc, const int* slot, void* d_ws,
-                                 size_t ws_bytes, void* stream) {
-  if (!op || op->kind != 1 || !d_psi || nint < 1 || !h_int || !weights || !nterms || !coef || !cls || !d_acc ||
-      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || !(half_width > 0.0))
+extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro, double shift,
+                                 double half_width, int ncls, const int* nterms, int kmax, const double* coef,
+                                 const void* d_G, const int* cls, const double* mult, const void* d_pairs,
+                                 const int* seg, void* d_acc, const int* slot, void* d_ws, size_t ws_bytes,
+                                 void* stream) {
+  if (!op || op->kind != 1 || !d_psi || nmacro < 1 || !nterms || !coef || !d_G || !cls || !mult || !d_acc ||
+      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || kmax > 127 || !(half_width > 0.0))
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: bad arguments");
-  if (d_pairs && !seg) return fail(RPM_E_INVALID, "rpm_adjoint_sweep: per-interval segments needed with pairs");
+  if (d_pairs && !seg) return fail(RPM_E_INVALID, "rpm_adjoint_sweep: per-step segments needed with pairs");
   for (int c = 0; c < ncls; ++c)
     if (nterms[c] < 2 || nterms[c] > kmax) return fail(RPM_E_INVALID, "Chebyshev length %d outside 2..%d", nterms[c], kmax);
-  for (int k = 0; k < nint; ++k)
+  for (int k = 0; k < nmacro; ++k)
     if (cls[k] < 0 || cls[k] >= ncls || slot[k] < 0 || (seg && seg[k] < 0))
-      return fail(RPM_E_INVALID, "rpm_adjoint_sweep: interval %d has a bad class, slot or segment", k);
+      return fail(RPM_E_INVALID, "rpm_adjoint_sweep: step %d has a bad class, slot or segment", k);
   if (ws_bytes < rpm_adjoint_workspace_bytes(op, batch, kmax))
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: workspace too small");
   SweepCtx C;
@@ -266,6 +269,7 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
   C.hs = half_width;
   C.s = static_cast<hipStream_t>(stream);
   const size_t vec = ((static_cast<size_t>(C.n) * batch * sizeof(c128)) + 255) & ~size_t(255);
+  const size_t qvec = ((static_cast<size_t>(C.nn) * batch * sizeof(c128)) + 255) & ~size_t(255);
   char* base = static_cast<char*>(d_ws);
   auto vecp = [&](int i) { return reinterpret_cast<c128*>(base + vec * i); };
   c128* psi_a = static_cast<c128*>(d_psi);
@@ -274,36 +278,67 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
   c128* g_b = vecp(2);
   c128* ra = vecp(3);
   c128* rb = vecp(4);
-  std::vector<c128*> u(kmax);
-  for (int i = 1; i < kmax; ++i) u[i] = vecp(4 + i);
-  C.dens_ws = reinterpret_cast<c128*>(base + vec * (5 + kmax));
-  C.dens_bytes = ws_bytes - vec * (5 + kmax);
+  c128* z = vecp(5);
+  std::vector<c128*> u(kmax + 1);
+  for (int i = 1; i <= kmax; ++i) u[i] = vecp(5 + i);
+  c128* chi = reinterpret_cast<c128*>(base + vec * (6 + kmax));
+  const long long cstride = static_cast<long long>(qvec / sizeof(c128));   // per order j
+  C.dens_ws = reinterpret_cast<c128*>(base + vec * (6 + kmax) + qvec * (kmax + 1));
+  C.dens_bytes = ws_bytes - (vec * (6 + kmax) + qvec * (kmax + 1));
+  if (C.dens_bytes < rpm_cross_density_workspace_bytes(C.nn, batch))
+    return fail(RPM_E_INVALID, "rpm_adjoint_sweep: workspace too small");
   RPM_HIP_CHECK(hipMemsetAsync(g_a, 0, static_cast<size_t>(C.n) * batch * sizeof(c128), C.s));
   const c128* pairs0 = static_cast<const c128*>(d_pairs);
+  const c128* G0 = static_cast<const c128*>(d_G);
   c128* acc0 = static_cast<c128*>(d_acc);
+  const double s2 = 2.0 / C.hs, a2 = -2.0 * C.shift / C.hs;
+  const int nqb = ceil_div(C.nn, 256);
   int rc = RPM_OK;
-  for (int k = nint; k >= 1; --k) {
-    const int iv = k - 1;                     // interval [t_{k-1}, t_k]
-    const int K = nterms[cls[iv]];
-    const double* cf = coef + static_cast<size_t>(cls[iv]) * (kmax + 1) * 2;
-    const c128* pairs = pairs0 ? pairs0 + static_cast<long long>(seg[iv]) * batch * 16 : nullptr;
-    c128* acc = acc0 + static_cast<long long>(slot[iv]) * batch * 16;
-    // costate source of the output at t_k: g += 2 W_k P psi_k
-    if ((rc = rpm_singlet_kick(psi_a, g_a, batch, C.n, C.nn, 2.0 * weights[k], C.s))) return rc;
-    // psi_{k-1} = U^H psi_k
-    if ((rc = clenshaw(C, psi_a, cf, K, pairs, nullptr, nullptr, ra, rb, psi_b))) return rc;
-    // u_n = T_n(X) psi_{k-1}, n < K
+  for (int k = nmacro - 1; k >= 0; --k) {
+    const int K = nterms[cls[k]];
+    const double* cf = coef + static_cast<size_t>(cls[k]) * (kmax + 1) * 2;
+    const c128* G = G0 + static_cast<size_t>(cls[k]) * (kmax + 1) * (kmax + 1);
+    const c128* pairs = pairs0 ? pairs0 + static_cast<long long>(seg[k]) * batch * 16 : nullptr;
+    c128* acc = acc0 + static_cast<long long>(slot[k]) * batch * 16;
+    // psi at the start of the step: psi_a = U^H psi_b
+    if ((rc = clenshaw_state(C, psi_a, cf, K, pairs, ra, rb, psi_b))) return rc;
+    // u_j = T_j(X) psi_a, j <= K, and their singlet parts
     u[0] = psi_b;
     if ((rc = affine(C, u[0], u[1], pairs, 1.0 / C.hs, -C.shift / C.hs, nullptr, cmake(0.0, 0.0), nullptr, 0.0)))
       return rc;
-    for (int n = 1; n + 1 < K; ++n)
-      if ((rc = affine(C, u[n], u[n + 1], pairs, 2.0 / C.hs, -2.0 * C.shift / C.hs, nullptr, cmake(0.0, 0.0), u[n - 1],
-                       -1.0)))
-        return rc;
-    // g_{k-1}' = U^H g_k with the interval's derivative densities
-    if ((rc = clenshaw(C, g_a, cf, K, pairs, u.data(), acc, ra, rb, g_b))) return rc;
+    for (int n = 1; n < K; ++n)
+      if ((rc = affine(C, u[n], u[n + 1], pairs, s2, a2, nullptr, cmake(0.0, 0.0), u[n - 1], -1.0))) return rc;
+    for (int j = 0; j <= K; ++j) {
+      hipLaunchKernelGGL(k_singlet_diff, dim3(nqb, batch), dim3(256), 0, C.s, u[j], chi + j * cstride, C.n, C.nn,
+                         C.nn);
+      RPM_LAUNCH_CHECK();
+    }
+    // Clenshaw over the costate with vector seeds z_m = conj(c_m) g + mult P sum_j G[m][j] u_j
+    auto seed = [&](int m) {
+      hipLaunchKernelGGL(k_seed, dim3(nqb, batch), dim3(256), 0, C.s, g_a, conjc(cf + 2 * m), chi, cstride, C.nn,
+                         K + 1, G + static_cast<size_t>(m) * (kmax + 1), mult[k], z, C.n, C.nn);
+      return hipGetLastError() == hipSuccess ? RPM_OK : fail(RPM_E_HIP, "k_seed launch failed");
+    };
+    if ((rc = seed(K))) return rc;
+    c128* hi = ra;   // nu_{n+2}
+    c128* lo = rb;   // nu_{n+1}
+    RPM_HIP_CHECK(hipMemcpyAsync(hi, z, static_cast<size_t>(C.n) * batch * sizeof(c128), hipMemcpyDeviceToDevice,
+                                 C.s));                                                  // nu_K
+    if ((rc = density(C, hi, u[K - 1], (K - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+    if ((rc = seed(K - 1))) return rc;
+    if ((rc = affine(C, hi, lo, pairs, s2, a2, z, cmake(1.0, 0.0), nullptr, 0.0))) return rc;   // nu_{K-1}
+    if ((rc = density(C, lo, u[K - 2], (K - 2 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+    for (int n = K - 2; n >= 1; --n) {
+      if ((rc = seed(n))) return rc;
+      if ((rc = affine(C, lo, hi, pairs, s2, a2, z, cmake(1.0, 0.0), hi, -1.0))) return rc;     // nu_n
+      if ((rc = density(C, hi, u[n - 1], (n - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+      std::swap(hi, lo);
+    }
+    // lo = nu_1, hi = nu_2: costate carried back to the start of the step
+    if ((rc = seed(0))) return rc;
+    if ((rc = affine(C, lo, g_b, pairs, 1.0 / C.hs, -C.shift / C.hs, z, cmake(1.0, 0.0), hi, -1.0))) return rc;
     std::swap(g_a, g_b);
-    std::swap(psi_a, psi_b);                  // psi_{k-1} becomes the current state
+    std::swap(psi_a, psi_b);
   }
   if (psi_a != d_psi)
     RPM_HIP_CHECK(hipMemcpyAsync(d_psi, psi_a, static_cast<size_t>(C.n) * batch * sizeof(c128),
