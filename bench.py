@@ -9,10 +9,11 @@ The same JSON line carries the other two parts of the metric from short
 probes: "batched" (32 trajectories = 8 field orientations x 4 random-phase
 vectors through the reference's adaptive Lanczos rules, against the plain
 one-trajectory-at-a-time path) and "gradient" (yield plus adjoint gradient
-over yield alone).
+over yield alone), plus "spectral" (the factorized J = D = 0 oracle's 199-angle
+curve on the GPU against the oracle's per-orientation CPU time).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
-    python bench.py --mode propagate|batch|grad|copy ...
+    python bench.py --mode propagate|batch|grad|spectral|copy ...
 """
 
 from __future__ import annotations
@@ -182,6 +183,75 @@ def grad_probe(h, args, B=None, horizon=None) -> dict:
             "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield, "psi0_recovery_error": rec}
 
 
+def spectral_cpu_sample(n_angles: int = 3) -> dict:
+    """Seconds per orientation of the oracle's factorized route (one thread), on a few angles."""
+    code = r"""
+import json, sys, time
+sys.path.insert(0, %r)
+from oracle import model, spectral
+fad = spectral.build_radical("FAD", model.FAD_NUCLEI)
+trp = spectral.build_radical("TrpH", model.TRP_NUCLEI)
+thetas = [0.0, 45.0, 90.0, 135.0, 180.0][:%d]
+t0 = time.perf_counter()
+for t in thetas:
+    spectral.solve_orientation(fad, trp, t, 50.0, (1.0, 2.0, 5.0, 10.0, 20.0, 100.0), 8.0)
+print(json.dumps({"n": len(thetas), "dt": time.perf_counter() - t0}))
+""" % (ROOT, n_angles)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               NUMEXPR_NUM_THREADS="1")
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, check=True)
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    return {"seconds_per_orientation": r["dt"] / r["n"], "cores": 1, "kind": "port",
+            "sample": f"{r['n']} orientations of the oracle's factorized route (Q = 0, 50 uT, six lifetimes), 1 thread"}
+
+
+def spectral_probe(args, with_cpu=True) -> dict:
+    """The 199-angle factorized Fig. 1B curve (J = D = 0, Q = 0, 50 uT, 6 lifetimes) on the GPU."""
+    import torch
+    from rpm_amd import model, spectral
+    thetas = model.theta_grid()
+    spectral.solve_orientations(thetas[:4], 50.0, model.DEFAULT_LIFETIMES_US, 8.0, quadrupole_scale=0.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rows = spectral.solve_orientations(thetas, 50.0, model.DEFAULT_LIFETIMES_US, 8.0, quadrupole_scale=0.0)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    out = {"mode": "spectral", "orientations": len(rows), "lifetimes": len(model.DEFAULT_LIFETIMES_US),
+           "seconds": dt, "orientations_per_s": len(rows) / dt,
+           "max_initial_singlet_error": max(abs(r["initial_singlet"] - 1.0) for r in rows)}
+    # the line-binning kernel alone: one launch = 32 FAD orientations x 6 pairs, grid stride of the curve
+    fad, trp = spectral.reference_models(0.0)
+    sa = spectral.diagonalize_radical(fad, thetas[:32], 50.0)
+    sb = spectral.diagonalize_radical(trp, thetas[:32], 50.0)
+    delta = (1.0 / max(model.DEFAULT_LIFETIMES_US)) / 8.0
+    ha = spectral.half_width_bins(float(sa.max_gap.max()), delta)
+    hb = spectral.half_width_bins(float(sb.max_gap.max()), delta)
+    stride = spectral._fast_length(2 * ha + 2 * hb + 1)
+    buf = torch.empty((32, len(spectral.PAIRS), stride), dtype=torch.complex128, device="cuda")
+    flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+    spectral.binned_spectra(sa, delta, ha, stride, out=buf, overflow=flag)
+    reps = 10
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        spectral.binned_spectra(sa, delta, ha, stride, out=buf, overflow=flag)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    d = fad.dimension
+    nbytes = buf.numel() * 16 + 32 * 3 * d * d * 16 + 32 * d * 8   # grid written once + T_x,y,z + eigenvalues read
+    out["bin_kernel"] = {"kernel": "k_bin_tiles", "orientations": 32, "radical": "FAD (d=576)", "grid_stride": stride,
+                         "ms_per_launch": ms, "bytes_per_launch": nbytes, "achieved_GBps": nbytes / ms / 1e6,
+                         "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBS}
+    del buf, sa, sb
+    if with_cpu:
+        cpu = spectral_cpu_sample()
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu_core"] = cpu["seconds_per_orientation"] * len(rows) / dt
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,7 +263,7 @@ def main() -> int:
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
-    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad"),
+    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -219,6 +289,9 @@ def main() -> int:
         return 0
     if args.mode == "grad":
         print(json.dumps(grad_probe(h, args)), flush=True)
+        return 0
+    if args.mode == "spectral":
+        print(json.dumps(spectral_probe(args, with_cpu=not args.no_cpu)), flush=True)
         return 0
     h._device_op().set_kernel(args.kernel)
     n = h.dimension
@@ -288,6 +361,8 @@ def main() -> int:
         result["batched"] = batch_probe(h, args, B=32, horizon=0.25, plain_sample=4)
         print("[bench] gradient probe", file=sys.stderr, flush=True)
         result["gradient"] = grad_probe(h, args, B=8, horizon=0.05)
+        print("[bench] factorized spectral probe", file=sys.stderr, flush=True)
+        result["spectral"] = spectral_probe(args, with_cpu=not args.no_cpu)
     print(json.dumps(result), flush=True)
     return 0
 

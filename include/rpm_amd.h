@@ -38,6 +38,11 @@
  *   rpm_operator_apply_affine  Chebyshev/Clenshaw step of the adjoint sweep (no reference counterpart: the
  *                              reference has no gradients; dH/dp is its Zeeman form R/poster_reproduction/core.py:186-194)
  *   rpm_cross_density          electron 4x4 of <lambda| dH/dp |psi> for electron-only dH/dp (no reference counterpart)
+ *
+ * Factorized spectral oracle (J = D = 0):
+ *   rpm_eigh_batched           diagonalize_radical's eigh, batched       R/rpm_quantum_needle_reproduction.py:127-142
+ *   rpm_bin_lines              transition_lines + binned_spectrum        R/rpm_quantum_needle_reproduction.py:145-172
+ *   rpm_spectral_yields        convolution + yields_from_spectrum sums   R/rpm_quantum_needle_reproduction.py:175-214
  */
 #ifndef RPM_AMD_H
 #define RPM_AMD_H
@@ -218,6 +223,42 @@ int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro
                       const void* d_G, const int* cls, const double* mult, const void* d_pairs,
                       const int* seg, void* d_acc, const int* slot, void* d_workspace, size_t workspace_bytes,
                       void* stream);
+
+/*
+ * Factorized spectral yields (R/rpm_quantum_needle_reproduction.py:145-172, transition_lines and
+ * binned_spectrum): the transition lines of one radical, frequency lambda_i - lambda_j and weight
+ * Tp[i][j] conj(Tq[i][j]) / nuclear_dimension, deposited by linear interpolation at grid index
+ * (lambda_i - lambda_j) / delta + half, for the six pairs (p, q) = (0,0) (1,1) (2,2) (0,1) (0,2) (1,2).
+ * d_eigenvalues: norient x d, ascending per orientation. d_ops: norient x 3 x d x d complex (T_x, T_y,
+ * T_z in the eigenbasis, row-major, Hermitian). d_grid: norient x 6 x grid_stride complex, written in
+ * full: bins 0 .. 2 half hold the spectrum, the rest zeros. If a line falls outside bins 0 .. 2 half,
+ * *d_overflow (device int) is incremented.
+ */
+int rpm_bin_lines(const double* d_eigenvalues, const void* d_ops, int d, int norient, int nuclear_dimension,
+                  double delta, long long half, void* d_grid, long long grid_stride, int* d_overflow, void* stream);
+
+/*
+ * Yields of the convolved spectra by Parseval (R/rpm_quantum_needle_reproduction.py:175-214, the FFT
+ * product, inverse FFT and yields_from_spectrum sums): for orientation b and lifetime t,
+ *   partial[blk][b][t] summed over blk = sum_f Re(cd_b(f) LD_t(f) + co_b(f) LO_t(f)),
+ * cd_b = sum over pairs 0..2 of FA_b,k FB_b,k and co_b over pairs 3..5 (the rpm_bin_lines order).
+ * d_fa, d_fb: norient x 6 x n complex (FFTs of the two radicals' binned spectra); d_ld, d_lo:
+ * nlifetimes x n complex (inverse FFTs of K_t and of 2 Re K_t on the output grid, zero beyond it).
+ * d_partial: nblocks x norient x nlifetimes doubles, written. nlifetimes <= 8.
+ */
+int rpm_spectral_yields(const void* d_fa, const void* d_fb, const void* d_ld, const void* d_lo, int norient,
+                        long long n, int nlifetimes, int nblocks, double* d_partial, void* stream);
+
+/*
+ * Batched Hermitian eigen-decomposition of row-major n x n complex matrices d_a (batch of them,
+ * contiguous), in place: on return d_a[b] holds V^H (row k = conjugated eigenvector k) and
+ * d_w[b*n + k] the eigenvalues in ascending order. method 0: divide and conquer (heevd), 1: Jacobi
+ * (heevdj). The per-matrix status (0 = converged) is left in the first batch ints of d_ws.
+ * Replaces scipy.linalg.eigh(..., driver="evd") of R/rpm_quantum_needle_reproduction.py:135-140,
+ * one matrix per orientation, with one call for the batch.
+ */
+size_t rpm_eigh_workspace_bytes(int n, int batch);
+int rpm_eigh_batched(void* d_a, int n, int batch, double* d_w, void* d_ws, size_t ws_bytes, int method, void* stream);
 
 /*
  * Embedded error estimates for a ladder of candidate dimensions.

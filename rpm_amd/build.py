@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "_lib")
 LIBNAME = "librpm_amd.so"
-SOURCES = ("errors.cpp", "operators.hip", "pair_tile.hip", "lanczos.hip", "krylov.hip", "adjoint.hip")
+SOURCES = ("errors.cpp", "operators.hip", "pair_tile.hip", "lanczos.hip", "krylov.hip", "adjoint.hip", "spectral.hip")
 ARCH = "gfx950"
 
 
@@ -56,7 +56,9 @@ def build(verbose: bool = False) -> str:
         objs = list(pool.map(compile_one, SOURCES))
     out = os.path.join(LIBDIR, LIBNAME)
     tmp = out + ".tmp"
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    # rocSOLVER (with rocBLAS) serves the batched Hermitian eigen-decomposition of the spectral route
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas",
+           "-Wl,-rpath,/opt/rocm/lib", "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stdout}\n{res.stderr}")

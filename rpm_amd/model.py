@@ -15,6 +15,8 @@ import numpy as np
 TWOPI = 2.0 * np.pi
 GAMMA_E_MHZ_PER_MT = 28.0
 DEFAULT_FIELD_MICROT = 50.0
+DEFAULT_LIFETIMES_US = (1.0, 2.0, 5.0, 10.0, 20.0, 100.0)
+NOMINAL_QUADRUPOLE_SCALE = 1.0
 MT_TO_RAD_PER_US = TWOPI * GAMMA_E_MHZ_PER_MT
 
 
@@ -139,3 +141,11 @@ def field_vector_mT(theta_deg: float, field_microT: float = DEFAULT_FIELD_MICROT
     phi = np.deg2rad(phi_deg)
     return 1e-3 * field_microT * np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi),
                                            np.cos(theta)])
+
+
+def theta_grid(step_deg: float = 1.0, fine_step_deg: float = 0.25, fine_half_width_deg: float = 3.0) -> np.ndarray:
+    """Field angles (deg): 0..180 in step_deg, plus fine_step_deg within +-fine_half_width_deg of 90
+    (R/rpm_quantum_needle_model.py:202-213; 199 angles by default)."""
+    coarse = np.arange(0.0, 180.0 + 0.5 * step_deg, step_deg)
+    fine = np.arange(90.0 - fine_half_width_deg, 90.0 + fine_half_width_deg + 0.5 * fine_step_deg, fine_step_deg)
+    return np.unique(np.round(np.concatenate([coarse, fine]), 10))
