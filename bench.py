@@ -10,10 +10,11 @@ probes: "batched" (32 trajectories = 8 field orientations x 4 random-phase
 vectors through the reference's adaptive Lanczos rules, against the plain
 one-trajectory-at-a-time path) and "gradient" (yield plus adjoint gradient
 over yield alone), plus "spectral" (the factorized J = D = 0 oracle's 199-angle
-curve on the GPU against the oracle's per-orientation CPU time).
+curve on the GPU against the oracle's per-orientation CPU time) and "sobol" (16 batched
+Sobol reaction-time cells on the full system).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
-    python bench.py --mode propagate|batch|grad|spectral|copy ...
+    python bench.py --mode propagate|batch|grad|spectral|sobol|copy ...
 """
 
 from __future__ import annotations
@@ -183,6 +184,29 @@ def grad_probe(h, args, B=None, horizon=None) -> dict:
             "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield, "psi0_recovery_error": rec}
 
 
+def sobol_probe(args, cells: int = 16, q_points: int = 16) -> dict:
+    """Batched Sobol reaction-time cells on the full system: tau = 1 us, Q points x 8 scrambles each,
+    angles spread over 0..180 deg, one batched propagation."""
+    from dataclasses import replace
+    import numpy as np
+    import torch
+    from rpm_amd import sobol
+    cfg = replace(sobol.RunConfig(), lifetimes_us=(1.0,))
+    tasks = [sobol.CellTask(float(t), i, q_points, False, cfg) for i, t in enumerate(np.linspace(0.0, 180.0, cells))]
+    sobol.run_cells([replace(tasks[0], q_points=1)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = sobol.run_cells(tasks, batch_size=cells)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    horizon = sum(float(sobol.reaction_time_schedule(cfg, q_points, t.sample_id).max()) for t in tasks)
+    return {"mode": "sobol", "cells": cells, "q_points": q_points, "scrambles": cfg.sobol_scrambles,
+            "lifetimes_us": list(cfg.lifetimes_us), "seconds": dt, "cells_per_s": cells / dt,
+            "reaction_times_per_s": cells * q_points * cfg.sobol_scrambles / dt,
+            "trajectory_us_per_s": horizon / dt, "matvecs": int(sum(r.matvecs for r in res)),
+            "mean_yield": float(np.mean([r.yields.mean() for r in res]))}
+
+
 def spectral_cpu_sample(n_angles: int = 3) -> dict:
     """Seconds per orientation of the oracle's factorized route (one thread), on a few angles."""
     code = r"""
@@ -263,7 +287,7 @@ def main() -> int:
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
-    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral"),
+    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -292,6 +316,9 @@ def main() -> int:
         return 0
     if args.mode == "spectral":
         print(json.dumps(spectral_probe(args, with_cpu=not args.no_cpu)), flush=True)
+        return 0
+    if args.mode == "sobol":
+        print(json.dumps(sobol_probe(args)), flush=True)
         return 0
     h._device_op().set_kernel(args.kernel)
     n = h.dimension
@@ -363,6 +390,8 @@ def main() -> int:
         result["gradient"] = grad_probe(h, args, B=8, horizon=0.05)
         print("[bench] factorized spectral probe", file=sys.stderr, flush=True)
         result["spectral"] = spectral_probe(args, with_cpu=not args.no_cpu)
+        print("[bench] Sobol reaction-time probe", file=sys.stderr, flush=True)
+        result["sobol"] = sobol_probe(args)
     print(json.dumps(result), flush=True)
     return 0
 

@@ -71,6 +71,29 @@ def pair_matrices(base, hamiltonians: Sequence) -> np.ndarray:
     return out
 
 
+def _trajectory_times(times_us, batch: int) -> list:
+    """One strictly increasing, non-negative time array per trajectory, from a shared 1-D sequence or
+    from `batch` sequences (which may differ in length)."""
+    try:
+        arr = np.asarray(times_us, dtype=float)
+    except (TypeError, ValueError):        # ragged: one sequence per trajectory
+        arr = None
+    if arr is not None and arr.ndim == 1:
+        lists = [arr] * batch
+    elif arr is not None and arr.ndim == 2:
+        lists = [arr[b] for b in range(arr.shape[0])]
+    else:
+        lists = [np.asarray(tuple(float(t) for t in x), dtype=float) for x in times_us]
+    if len(lists) != batch:
+        raise ValueError("times_us must be one sequence or one sequence per trajectory")
+    for t in lists:
+        if t.ndim != 1 or t.size == 0:
+            raise ValueError("At least one observation time is required.")
+        if np.any(t < 0.0) or np.any(np.diff(t) <= 0.0):
+            raise ValueError("Observation times must be nonnegative and strictly increasing.")
+    return lists
+
+
 class BatchedLanczosPropagator:
     """Adaptive Lanczos propagation of a batch of trajectories with the singlet observable.
 
@@ -242,6 +265,8 @@ class BatchedLanczosPropagator:
         """Propagate B trajectories; returns one ``PropagationResult`` per trajectory.
 
         ``initial_states``: (B, N) NumPy array or complex128 GPU tensor (copied).
+        ``times_us``: one increasing sequence shared by the batch, or B of them (each trajectory
+        then follows the step rules for its own final time, as a separate run would).
         ``pairs``: optional (B, 4, 4) electron matrices (see ``pair_matrices``);
         default: the Hamiltonian's own.  ``schedules``: optional per-trajectory
         recorded schedules to replay (entries may be None).
@@ -254,11 +279,6 @@ class BatchedLanczosPropagator:
         h = self.hamiltonian
         op = h._device_op()
         n = h.dimension
-        times = np.asarray(tuple(float(t) for t in times_us), dtype=float)
-        if times.ndim != 1 or times.size == 0:
-            raise ValueError("At least one observation time is required.")
-        if np.any(times < 0.0) or np.any(np.diff(times) <= 0.0):
-            raise ValueError("Observation times must be nonnegative and strictly increasing.")
         x0 = initial_states
         if nat.is_torch(x0):
             states = x0.to(device=dev, dtype=torch.complex128).reshape(x0.shape[0], -1).clone()
@@ -270,6 +290,7 @@ class BatchedLanczosPropagator:
             raise ValueError("Initial state dimension does not match the Hamiltonian.")
         if B < 1 or B > 65535:
             raise ValueError("batch size must be in 1..65535")
+        traj_times = _trajectory_times(times_us, B)
         m = min(self.maximum_dimension, n)
         if m > 64:
             raise ValueError("maximum_dimension above 64 is not supported by the device kernels.")
@@ -320,7 +341,7 @@ class BatchedLanczosPropagator:
         for b in range(B):
             replay = None if schedules is None or schedules[b] is None else list(schedules[b])
             call = None if cb is None else (lambda *a, b=b: cb(b, *a))
-            gens[b] = self._rules(times, replay, call)
+            gens[b] = self._rules(traj_times[b], replay, call)
             pending[b] = next(gens[b])
 
         def resume(b, value):
@@ -416,5 +437,5 @@ class BatchedLanczosPropagator:
         out = []
         for b in range(B):
             values, bounds, errors, stats, record = results[b]
-            out.append(PropagationResult(times, values, bounds, errors, None, None, stats, record, states[b]))
+            out.append(PropagationResult(traj_times[b], values, bounds, errors, None, None, stats, record, states[b]))
         return out

@@ -206,11 +206,20 @@ class MatrixFreeHamiltonian(_GpuHamiltonianBase):
         self.matvec_count = 0
         self._dev = None
 
+    # Radical axis groups are inferred from the terms (electron-nucleus couplings) when the pair
+    # stencil is built, so none are configured here.
+    left_axes = None
+    right_axes = None
+
     def _build_device_operator(self):
         pair = _pair_form(self.dimensions, self.terms)
         if pair is not None:
             return _create_pair(*pair)
         return _create_local(self.dimensions, self.terms)
+
+    @property
+    def uses_pair_stencil(self) -> bool:
+        return _pair_form(self.dimensions, self.terms) is not None
 
 
 class CompiledFullPairHamiltonian(_GpuHamiltonianBase):

@@ -17,6 +17,8 @@ GAMMA_E_MHZ_PER_MT = 28.0
 DEFAULT_FIELD_MICROT = 50.0
 DEFAULT_LIFETIMES_US = (1.0, 2.0, 5.0, 10.0, 20.0, 100.0)
 NOMINAL_QUADRUPOLE_SCALE = 1.0
+PILOT_ANGLES_DEG = (0.0, 80.0, 90.0, 100.0, 180.0)
+DIAGNOSTIC_ANGLES_DEG = PILOT_ANGLES_DEG
 MT_TO_RAD_PER_US = TWOPI * GAMMA_E_MHZ_PER_MT
 
 
@@ -149,3 +151,24 @@ def theta_grid(step_deg: float = 1.0, fine_step_deg: float = 0.25, fine_half_wid
     coarse = np.arange(0.0, 180.0 + 0.5 * step_deg, step_deg)
     fine = np.arange(90.0 - fine_half_width_deg, 90.0 + fine_half_width_deg + 0.5 * fine_step_deg, fine_step_deg)
     return np.unique(np.round(np.concatenate([coarse, fine]), 10))
+
+
+def diagnostic_time_grid_us() -> np.ndarray:
+    """0..5 us every 0.01 us, then 257 log-spaced points to 500 us (R/rpm_quantum_needle_model.py:216-219)."""
+    dense = np.arange(0.0, 5.0 + 0.005, 0.01)
+    logarithmic = np.geomspace(5.0, 500.0, 257)
+    return np.unique(np.concatenate([dense, logarithmic]))
+
+
+def nominal_quadrupole_metadata(scale: float) -> dict:
+    """Configuration-record entries for active Q tensors; empty at scale 0 so that Q-off records keep
+    their fingerprints (R/rpm_quantum_needle_model.py:153-161)."""
+    scale = float(scale)
+    if not math.isfinite(scale) or scale < 0.0:
+        raise ValueError("quadrupole scale must be finite and non-negative")
+    if scale == 0:
+        return {}
+    return {
+        "nominal_quadrupole_tensors_mT": {k: q.tolist() for k, q in QUADRUPOLE_TENSORS_MT.items()},
+        "quadrupole_frame": "shared flavin molecular frame; N5 in-plane azimuth provisional",
+    }
