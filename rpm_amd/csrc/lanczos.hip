@@ -148,16 +148,23 @@ __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict
   }
 }
 
-// alpha[b*astride + j] = Re sum_u dots[b][u] (the matvec's per-unit partials of <v_j, H v_j>).
+// Re sum_u dots[b][u] (the matvec's per-unit partials of <v_j, H v_j>), fixed order; every thread
+// of the (kThreads-wide) block receives it.
+__device__ inline double alpha_sum(const c128* __restrict__ dots, int units, int b, c128* sm) {
+  c128 s{0.0, 0.0};
+  for (int i = threadIdx.x; i < units; i += kThreads) s = cadd(s, dots[static_cast<long long>(b) * units + i]);
+  return block_sum(s, sm).re;
+}
+
+// alpha[b*astride + j] = Re sum_u dots[b][u]. A single trajectory's basis forms alpha inside
+// k_gs_dots for all but its last step.
 __global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dots, int units, double* __restrict__ alpha,
                                                     int astride, int j, const BasisCtl* ctl) {
   const int b = blockIdx.x;
   if (!basis_live(ctl, b, j, 0)) return;
   __shared__ c128 sm[kThreads / kWave];
-  c128 s{0.0, 0.0};
-  for (int i = threadIdx.x; i < units; i += kThreads) s = cadd(s, dots[static_cast<long long>(b) * units + i]);
-  s = block_sum(s, sm);
-  if (threadIdx.x == 0) alpha[b * astride + j] = s.re;
+  const double a = alpha_sum(dots, units, b, sm);
+  if (threadIdx.x == 0) alpha[b * astride + j] = a;
 }
 
 // w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2; with
@@ -205,24 +212,42 @@ __global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V,
 // Fused pass over the basis: partial[c][blk] = sum_x conj(V_c[x]) w'[x] for all
 // c < k with w' = w - alpha_j V_j (j = k-1), the finished three-term step.  w'
 // is formed in registers only (every wave reads w); k_update subtracts
-// alpha_j V_j together with the projection.
+// alpha_j V_j together with the projection. With `adots`, every block first forms
+// alpha_j from the matvec's partials (block 0 stores it) instead of reading it.
 // Wave q of the 4-wave block owns columns c = q, q+4, ..., one element per lane
 // per iteration, so every lane keeps up to 16 column loads in flight and no LDS
 // staging is needed; each block covers `per` consecutive 64-element runs and
 // ends with one wave reduction per column.  k <= kMaxBasis.
 constexpr int kDotWaves = 4;
 constexpr int kDotCols = kMaxBasis / kDotWaves;
+// 64-element runs per k_gs_dots block. The same in batches, so a batched basis is bitwise the
+// single-trajectory basis.
+constexpr int kDotPer = 8;
+static_assert(kDotWaves * kWave == kThreads, "k_gs_dots shares the kThreads-wide reductions");
 __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __restrict__ V, long long vstride, long long n,
-                                                               int k, int per, const double* __restrict__ alpha,
-                                                               int astride, const c128* __restrict__ W,
+                                                               int k, int per, double* __restrict__ alpha,
+                                                               int astride, const c128* __restrict__ adots, int units,
+                                                               const c128* __restrict__ W,
                                                                c128* __restrict__ partial, long long pstride, int j,
                                                                int need, const BasisCtl* ctl) {
   const int b = blockIdx.y;
-  if (!basis_live(ctl, b, j, need)) return;
+  if (!basis_live(ctl, b, j, adots ? 0 : need)) return;
+  __shared__ c128 sm[kThreads / kWave];
+  double a;
+  if (adots) {
+    // alpha_j from the matvec's partials: every block forms the same sum in the same order as
+    // k_alpha (the partials come from the previous kernel, so no cross-block synchronisation is
+    // needed). Block 0 stores it, also when step j is the trajectory's last and the projection
+    // below is skipped.
+    a = alpha_sum(adots, units, b, sm);
+    if (blockIdx.x == 0 && threadIdx.x == 0) alpha[b * astride + j] = a;
+    if (!basis_live(ctl, b, j, need)) return;   // uniform over the blocks of vector b
+  } else {
+    a = alpha[b * astride + j];
+  }
   const c128* Vb = V + b * vstride;
   const c128* w = W + b * n;
   const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
-  const double a = alpha ? alpha[b * astride + j] : 0.0;
   const c128* vj = Vb + static_cast<long long>(k - 1) * n;
   c128 acc[kDotCols];
 #pragma unroll
@@ -232,7 +257,7 @@ __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __res
     const long long x = x0 + static_cast<long long>(it) * kWave;
     if (x >= n) break;
     c128 wv = w[x];
-    if (alpha) {
+    {
       const c128 v = vj[x];
       wv.re = fma(-a, v.re, wv.re);
       wv.im = fma(-a, v.im, wv.im);
@@ -443,7 +468,7 @@ struct BasisWs {
 
 BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   const int nchunk = ceil_div(n, kChunk);
-  const int gsb = ceil_div(n, 8 * kWave);   // dots blocks at the smallest block size
+  const int gsb = ceil_div(n, static_cast<long long>(kDotPer) * kWave);   // k_gs_dots blocks
   const int nupd = ceil_div(n, kUpdChunk);
   size_t off = 0;
   auto take = [&](size_t b) {
@@ -484,9 +509,7 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   const int nchunk = ceil_div(n, kChunk);
   const int nupd = ceil_div(n, kUpdChunk);
   const int count = static_cast<int>(max_target < n ? max_target : n);
-  // elements per dots block: 8 x 64 (enough blocks to fill the chip for one vector); the same in
-  // batches, so a batched basis is bitwise the single-trajectory basis
-  const int dot_per = 8;
+  const int dot_per = kDotPer;
   const int dot_blocks = ceil_div(n, static_cast<long long>(dot_per) * kWave);
   if (dot_blocks > w.nblk) return fail(RPM_E_INVALID, "internal: dots partial buffer too small");
   hipLaunchKernelGGL(k_sumsq, dim3(nchunk, batch), dim3(kThreads), 0, s, states, n, sld, w.sq, d_targets);
@@ -510,13 +533,19 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     A.dots_ld = w.units;
     int rc = operator_apply_ex(op, in, out, batch, j == 0 ? sld : n, A, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, alpha, m, j, w.ctl);
-    RPM_LAUNCH_CHECK();
+    // A single trajectory forms alpha_j inside k_gs_dots (one launch fewer per step). For a batch
+    // the per-block recomputation costs more than the launch, so k_alpha runs; both sum in the
+    // same order, so the bases stay bitwise equal.
+    const bool fuse_alpha = batch == 1 && j < count - 1;
+    if (!fuse_alpha) {
+      hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, alpha, m, j, w.ctl);
+      RPM_LAUNCH_CHECK();
+    }
     if (j == count - 1) break;
     const int k = j + 1;
-    // pass 1: w -= alpha_j V[j], h = V^H w
+    // pass 1: (alpha_j,) w -= alpha_j V[j], h = V^H w
     hipLaunchKernelGGL(k_gs_dots, dim3(dot_blocks, batch), dim3(kDotWaves * kWave), 0, s, V, vstride, n, k, dot_per,
-                       alpha, m, out, w.partial, w.pstride, j, 1, w.ctl);
+                       alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, dot_blocks, w.pstride, w.h, m,
                        nullptr, m, 0, j, 1, w.ctl);
