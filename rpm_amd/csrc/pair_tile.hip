@@ -207,22 +207,32 @@ __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6
 k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, long long ld, int batch, ApplyCtl A) {
   constexpr int LG = 1 << NC;
   constexpr int NT = NR * LG;
+  static_assert(NR % kWave == 0, "a wave must hold a single left index");
+  static_assert(KR % 4 == 0, "right neighbour descriptors are fetched four at a time");
   __shared__ c128 tile[4 * LG * NR];
   __shared__ c128 coefT[KR * 3 * 4];
+  __shared__ c128 tabB[4 * NR];   // dB[0], dB[1], fB[0], fB[1] over r
   int b, grp;
   tile_map(T.ngroups, batch, b, grp);
   double s;
   if (!tile_begin(A, b, s)) return;
-  PairCoef E;
-  load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
   const int tid = threadIdx.x;
-  const int l = tid / NR, r = tid - (tid / NR) * NR;
+  // The vector-memory path (address and data units), not HBM, limits this kernel, so every
+  // per-thread load that is not amplitude data is taken off it: l is wave-uniform (NR is a
+  // multiple of the wave size), so the left tables are scalar loads; the right tables are staged
+  // once per block in LDS; the right neighbour descriptors come in 16-byte pieces.
+  const int l = __builtin_amdgcn_readfirstlane(tid / NR);
+  const int r = tid - l * NR;
   const long long S = T.NLR;
   const c128* __restrict__ x = in + b * ld;
   const int l0 = grp * LG;
   const long long own = static_cast<long long>(l0 + l) * NR + r;
 
   for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
+  for (int i = tid; i < 4 * NR; i += NT) {
+    const int q = i / NR;
+    tabB[i] = (q < 2 ? T.dB : T.fB)[(q & 1) * NR + (i - q * NR)];
+  }
   c128 psi[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
@@ -230,23 +240,10 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     tile[(e * LG + l) * NR + r] = psi[e];
   }
 
-  c128 acc[4];
-  {
-    const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
-    const c128 fA0 = T.fA[l0 + l], fA1 = T.fA[T.NL + l0 + l];
-    const c128 dB0 = T.dB[r], dB1 = T.dB[NR + r], fB0 = T.fB[r], fB1 = T.fB[NR + r];
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), E.d[e]);
-      c128 a = cfma(d, psi[e], cmake(0.0, 0.0));
-      a = cfma(cadd((e >> 1) ? fA1 : fA0, E.fa[e]), psi[e ^ 2], a);
-      a = cfma(cadd((e & 1) ? fB1 : fB0, E.fb[e]), psi[e ^ 1], a);
-      a = cfma(E.bo[e], psi[e ^ 3], a);
-      acc[e] = a;
-    }
-  }
-
   // open left axes (global / L2): offsets and coefficients uniform over the tile
+  c128 acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = cmake(0.0, 0.0);
   {
     const int* __restrict__ ofs = T.ofsO + grp * KO;
     const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
@@ -262,6 +259,22 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   }
 
   __syncthreads();
+  {
+    PairCoef E;
+    load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
+    const c128 dA0 = T.dA[l0 + l], dA1 = T.dA[T.NL + l0 + l];
+    const c128 fA0 = T.fA[l0 + l], fA1 = T.fA[T.NL + l0 + l];
+    const c128 dB0 = tabB[r], dB1 = tabB[NR + r], fB0 = tabB[2 * NR + r], fB1 = tabB[3 * NR + r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), E.d[e]);
+      c128 a = cfma(d, psi[e], acc[e]);
+      a = cfma(cadd((e >> 1) ? fA1 : fA0, E.fa[e]), psi[e ^ 2], a);
+      a = cfma(cadd((e & 1) ? fB1 : fB0, E.fb[e]), psi[e ^ 1], a);
+      a = cfma(E.bo[e], psi[e ^ 3], a);
+      acc[e] = a;
+    }
+  }
 #pragma unroll
   for (int j = 0; j < NC; ++j) {
     const int l2 = l ^ (1 << j);
@@ -271,9 +284,18 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
     mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, q, acc);
   }
+  int pkr[KR];
+#pragma unroll
+  for (int k4 = 0; k4 < KR / 4; ++k4) {
+    const int4 v = reinterpret_cast<const int4*>(T.packR + r * KR)[k4];
+    pkr[4 * k4] = v.x;
+    pkr[4 * k4 + 1] = v.y;
+    pkr[4 * k4 + 2] = v.z;
+    pkr[4 * k4 + 3] = v.w;
+  }
 #pragma unroll
   for (int k = 0; k < KR; ++k) {
-    const int pk = T.packR[r * KR + k];
+    const int pk = pkr[k];
     const int off = static_cast<short>(pk & 0xffff);
     const int m = pk >> 16;
     c128 C[4], q[4];
