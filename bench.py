@@ -292,7 +292,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(trajectories shard independently across GPUs; see DESIGN.md).", file=sys.stderr)
+              "(trajectories are independent; a multi-GPU run gives each GPU its own block of them).", file=sys.stderr)
         return 2
 
     import numpy as np

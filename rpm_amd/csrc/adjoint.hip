@@ -12,6 +12,11 @@
 
 namespace rpm {
 
+__global__ void __launch_bounds__(256) k_zero(c128* __restrict__ x, long long n) {
+  const long long i = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < n) x[i] = c128{0.0, 0.0};
+}
+
 constexpr int kDensThreads = 256;
 constexpr int kDensPer = 8;   // nuclear indices per thread
 
@@ -287,7 +292,13 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
   C.dens_bytes = ws_bytes - (vec * (6 + kmax) + qvec * (kmax + 1));
   if (C.dens_bytes < rpm_cross_density_workspace_bytes(C.nn, batch))
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: workspace too small");
-  RPM_HIP_CHECK(hipMemsetAsync(g_a, 0, static_cast<size_t>(C.n) * batch * sizeof(c128), C.s));
+  {
+    // a fill kernel, not hipMemsetAsync: memset nodes misbehave on replay when a caller captures
+    // this entry point in a HIP graph
+    const long long total = C.n * static_cast<long long>(batch);
+    hipLaunchKernelGGL(k_zero, dim3(ceil_div(total, 256)), dim3(256), 0, C.s, g_a, total);
+    RPM_LAUNCH_CHECK();
+  }
   const c128* pairs0 = static_cast<const c128*>(d_pairs);
   const c128* G0 = static_cast<const c128*>(d_G);
   c128* acc0 = static_cast<c128*>(d_acc);
