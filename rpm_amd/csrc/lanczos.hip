@@ -224,6 +224,10 @@ constexpr int kDotCols = kMaxBasis / kDotWaves;
 // single-trajectory basis.
 constexpr int kDotPer = 8;
 static_assert(kDotWaves * kWave == kThreads, "k_gs_dots shares the kThreads-wide reductions");
+// CPW columns per wave: block z covers columns [z * 4 CPW, (z + 1) * 4 CPW), column c in wave c % 4.
+// Whatever CPW, a column's partial is formed by the same lanes over the same elements in the same
+// order, so the result is bitwise the same; a small CPW only spreads one vector over more blocks.
+template <int CPW>
 __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __restrict__ V, long long vstride, long long n,
                                                                int k, int per, double* __restrict__ alpha,
                                                                int astride, const c128* __restrict__ adots, int units,
@@ -240,7 +244,7 @@ __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __res
     // needed). Block 0 stores it, also when step j is the trajectory's last and the projection
     // below is skipped.
     a = alpha_sum(adots, units, b, sm);
-    if (blockIdx.x == 0 && threadIdx.x == 0) alpha[b * astride + j] = a;
+    if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) alpha[b * astride + j] = a;
     if (!basis_live(ctl, b, j, need)) return;   // uniform over the blocks of vector b
   } else {
     a = alpha[b * astride + j];
@@ -249,9 +253,10 @@ __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __res
   const c128* w = W + b * n;
   const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
   const c128* vj = Vb + static_cast<long long>(k - 1) * n;
-  c128 acc[kDotCols];
+  const int c0 = static_cast<int>(blockIdx.z) * kDotWaves * CPW + q;
+  c128 acc[CPW];
 #pragma unroll
-  for (int i = 0; i < kDotCols; ++i) acc[i] = cmake(0.0, 0.0);
+  for (int i = 0; i < CPW; ++i) acc[i] = cmake(0.0, 0.0);
   const long long x0 = static_cast<long long>(blockIdx.x) * per * kWave + lane;
   for (int it = 0; it < per; ++it) {
     const long long x = x0 + static_cast<long long>(it) * kWave;
@@ -263,15 +268,15 @@ __global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __res
       wv.im = fma(-a, v.im, wv.im);
     }
 #pragma unroll
-    for (int i = 0; i < kDotCols; ++i) {
-      const int c = q + kDotWaves * i;
+    for (int i = 0; i < CPW; ++i) {
+      const int c = c0 + kDotWaves * i;
       if (c < k) acc[i] = cfmaconj(Vb[static_cast<long long>(c) * n + x], wv, acc[i]);
     }
   }
   const int nblk = gridDim.x;
 #pragma unroll
-  for (int i = 0; i < kDotCols; ++i) {
-    const int c = q + kDotWaves * i;
+  for (int i = 0; i < CPW; ++i) {
+    const int c = c0 + kDotWaves * i;
     if (c < k) {
       const c128 t = wave_sum(acc[i]);
       if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
@@ -544,8 +549,16 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     if (j == count - 1) break;
     const int k = j + 1;
     // pass 1: (alpha_j,) w -= alpha_j V[j], h = V^H w
-    hipLaunchKernelGGL(k_gs_dots, dim3(dot_blocks, batch), dim3(kDotWaves * kWave), 0, s, V, vstride, n, k, dot_per,
-                       alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1, w.ctl);
+    // one vector: 2 columns per wave over ceil(k / 8) column groups, so that it fills the chip;
+    // a batch: all columns in one block (same partials either way)
+    if (batch == 1)
+      hipLaunchKernelGGL(k_gs_dots<2>, dim3(dot_blocks, 1, ceil_div(k, kDotWaves * 2)), dim3(kDotWaves * kWave), 0, s,
+                         V, vstride, n, k, dot_per, alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
+                         w.pstride, j, 1, w.ctl);
+    else
+      hipLaunchKernelGGL(k_gs_dots<kDotCols>, dim3(dot_blocks, batch), dim3(kDotWaves * kWave), 0, s, V, vstride, n, k,
+                         dot_per, alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1,
+                         w.ctl);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, dot_blocks, w.pstride, w.h, m,
                        nullptr, m, 0, j, 1, w.ctl);
