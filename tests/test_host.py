@@ -72,6 +72,28 @@ def test_last_error_and_invalid_arguments_without_gpu():
     assert lib.rpm_lanczos_workspace_bytes(None, 8) == 0
 
 
+def test_spectral_entry_points_reject_bad_arguments_without_gpu():
+    """Argument checks run before any device call (null buffers, empty or oversized shapes)."""
+    lib = _native.load()
+    vp = ctypes.c_void_p
+    one = vp(1)
+    # rpm_bin_lines: null buffers, a stride shorter than the grid, a non-positive bin width
+    assert lib.rpm_bin_lines(None, one, 4, 1, 2, 0.1, 10, one, 21, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 2, 0.1, 10, one, 20, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 2, 0.0, 10, one, 21, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 0, 1, 2, 0.1, 10, one, 21, one, None) == _native.RPM_E_INVALID
+    # rpm_spectral_yields: more than 8 lifetimes per pass, empty batch
+    assert lib.rpm_spectral_yields(one, one, one, one, 2, 100, 9, 4, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_spectral_yields(one, one, one, one, 0, 100, 6, 4, one, None) == _native.RPM_E_INVALID
+    # rpm_eigh_batched: unknown method, workspace smaller than rpm_eigh_workspace_bytes
+    need = lib.rpm_eigh_workspace_bytes(8, 3)
+    assert need >= 3 * 4 + 8 * 3 * 8 and lib.rpm_eigh_workspace_bytes(0, 3) == 0
+    assert lib.rpm_eigh_batched(one, 8, 3, one, one, need, 2, None) == _native.RPM_E_INVALID
+    assert lib.rpm_eigh_batched(one, 8, 3, one, one, need - 1, 0, None) == _native.RPM_E_INVALID
+    with pytest.raises(ValueError):
+        _native.check(lib.rpm_eigh_batched(None, 8, 3, one, one, need, 0, None))
+
+
 def _expand_pair(dims, form):
     """Dense H from the pair-stencil tables (the kernel's formula, written with Kronecker products)."""
     ldims, rdims, blocks, pair = form
