@@ -223,6 +223,7 @@ constexpr int kDotCols = kMaxBasis / kDotWaves;
 // 64-element runs per k_gs_dots block. The same in batches, so a batched basis is bitwise the
 // single-trajectory basis.
 constexpr int kDotPer = 8;
+constexpr int kBatchCpw = 8;    // columns per wave of k_gs_dots in a batch (2 column groups at k = 64)
 static_assert(kDotWaves * kWave == kThreads, "k_gs_dots shares the kThreads-wide reductions");
 // CPW columns per wave: block z covers columns [z * 4 CPW, (z + 1) * 4 CPW), column c in wave c % 4.
 // Whatever CPW, a column's partial is formed by the same lanes over the same elements in the same
@@ -556,9 +557,9 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
                          V, vstride, n, k, dot_per, alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
                          w.pstride, j, 1, w.ctl);
     else
-      hipLaunchKernelGGL(k_gs_dots<kDotCols>, dim3(dot_blocks, batch), dim3(kDotWaves * kWave), 0, s, V, vstride, n, k,
-                         dot_per, alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1,
-                         w.ctl);
+      hipLaunchKernelGGL(k_gs_dots<kBatchCpw>, dim3(dot_blocks, batch, ceil_div(k, kDotWaves * kBatchCpw)),
+                         dim3(kDotWaves * kWave), 0, s, V, vstride, n, k, dot_per, alpha, m,
+                         fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1, w.ctl);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, dot_blocks, w.pstride, w.h, m,
                        nullptr, m, 0, j, 1, w.ctl);
