@@ -59,6 +59,69 @@ print(json.dumps({"n": n, "dt": dt}))
                       f"(production operator, dim 221184) over {r['dt']:.1f} s, 1 thread"}
 
 
+def cpu_extra_baselines(prefix_us: float = 0.1, all_core_seconds: float = 6.0) -> dict:
+    """The oracle on the host: (ii) single-core trajectory times, config 0 (576 levels, 5 us) and a
+    0.1 us full-system prefix with the horizon-scaled budget, extrapolated to the full horizon;
+    (iii) all-core compiled-pair H.psi throughput, one worker per core up to 16."""
+    code = r"""
+import json, math, sys, time
+sys.path.insert(0, %r)
+import numpy as np
+from oracle import core as ocore
+from rpm_amd import workloads
+out = {}
+sys0 = workloads.small_system(0)
+h0 = sys0.hamiltonian(builder=ocore.build_hamiltonian)
+cfg0 = ocore.SweepConfig(lifetime_us=1.0, survival_tail_bound=math.exp(-5.0))
+t0 = time.perf_counter()
+ocore.run_trajectory(h0, ocore.random_phase_initial_state(h0.dimensions, 0, 0), cfg0)
+out["config0_seconds"] = time.perf_counter() - t0
+hp = ocore.build_hamiltonian(90.0, 0.05)
+cfg = ocore.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+times = np.arange(int(round(%f / 0.00125)) + 1) * 0.00125
+prop = ocore.make_propagator(hp, cfg, state_error_budget=cfg.state_error_budget * float(times[-1]) / cfg.maximum_time_us)
+s = ocore.random_phase_initial_state(hp.dimensions, ocore.MASTER_SEED, 0)
+t0 = time.perf_counter()
+prop.propagate(s, times, ocore.SingletPopulationObservable(hp.dimensions))
+dt = time.perf_counter() - t0
+out["prefix_us"] = float(times[-1])
+out["prefix_seconds"] = dt
+out["full_horizon_us"] = cfg.maximum_time_us
+out["extrapolated_trajectory_seconds"] = dt * cfg.maximum_time_us / float(times[-1])
+print(json.dumps(out))
+""" % (ROOT, prefix_us)
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1",
+               NUMEXPR_NUM_THREADS="1")
+    res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, check=True)
+    out = json.loads(res.stdout.strip().splitlines()[-1])
+    workers = max(1, min(16, os.cpu_count() or 1))
+    worker = r"""
+import json, sys, time
+sys.path.insert(0, %r)
+import numpy as np
+from oracle import core
+h = core.build_hamiltonian(90.0, 0.05)
+v = np.random.default_rng(int(sys.argv[1])).normal(size=h.dimension) + 0j
+v /= np.linalg.norm(v)
+h.matvec(v)
+n, t0 = 0, time.perf_counter()
+while time.perf_counter() - t0 < %f:
+    v = h.matvec(v); v /= np.linalg.norm(v); n += 1
+print(json.dumps({"n": n, "dt": time.perf_counter() - t0}))
+""" % (ROOT, all_core_seconds)
+    procs = [subprocess.Popen([sys.executable, "-c", worker, str(i)], stdout=subprocess.PIPE, text=True, env=env)
+             for i in range(workers)]
+    rate = 0.0
+    for pr in procs:
+        o, _ = pr.communicate()
+        r = json.loads(o.strip().splitlines()[-1])
+        rate += r["n"] / r["dt"]
+    out["all_core_hpsi_per_s"] = rate
+    out["all_core_workers"] = workers
+    out["kind"] = "port"
+    return out
+
+
 def measured_traffic(batch: int):
     """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary, if present."""
     path = os.path.join(ROOT, "profiles", "hpsi_pair_traffic.json")
@@ -70,14 +133,29 @@ def measured_traffic(batch: int):
     return None if per_vector is None else per_vector * batch
 
 
-def propagate_probe(h, args) -> int:
+def lanczos_algorithmic_bytes(schedule, n: int) -> int:
+    """Algorithmic HBM bytes of a propagation under the reference's CGS2 semantics: per Krylov basis of
+    size m, 16 N (2m [matvec] + 2m(m+1) [two projections over the basis] + 7m [alpha, three-term step,
+    norm, scale]); per accepted step, 16 N (m+1) for the endpoint V c and 8 N m for the singlet
+    projection, m the selected dimension. Summed over the recorded schedule (accepted and rejected)."""
+    total = 0
+    for _, accepted, selected, sizes in schedule:
+        for m in sizes:
+            total += 16 * n * (2 * m + 2 * m * (m + 1) + 7 * m)
+        if accepted:
+            total += 16 * n * (selected + 1) + 8 * n * selected
+    return total
+
+
+def propagate_probe(h, args, horizon=None) -> dict:
     """One production trajectory over a short horizon with the horizon-scaled error budget
     of the reference's speed probe (R/rpm_full14_quadrupole_experiment.py:138-148)."""
     import numpy as np
     import torch
     from rpm_amd import core
+    horizon = args.horizon if horizon is None else horizon
     cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
-    times = np.arange(int(round(args.horizon / 0.00125)) + 1) * 0.00125
+    times = np.arange(int(round(horizon / 0.00125)) + 1) * 0.00125
     prop = core.make_propagator(h, cfg, probe_horizon=float(times[-1]))
     s = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 0)
     obs = core.SingletPopulationObservable(h.dimensions)
@@ -88,11 +166,14 @@ def propagate_probe(h, args) -> int:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = res.stats
-    print(json.dumps({"mode": "propagate", "horizon_us": float(times[-1]), "seconds": dt,
-                      "simulated_us_per_s": float(times[-1]) / dt, "accepted": st.accepted_steps,
-                      "rejected": st.rejected_steps, "matvecs": st.matvecs, "max_krylov": st.maximum_krylov_dimension,
-                      "ms_per_matvec_incl_cgs2": dt * 1e3 / st.matvecs}), flush=True)
-    return 0
+    nbytes = lanczos_algorithmic_bytes(res.schedule, h.dimension)
+    return {"mode": "propagate", "horizon_us": float(times[-1]), "seconds": dt,
+            "simulated_us_per_s": float(times[-1]) / dt, "accepted": st.accepted_steps,
+            "rejected": st.rejected_steps, "matvecs": st.matvecs, "max_krylov": st.maximum_krylov_dimension,
+            "ms_per_matvec_incl_cgs2": dt * 1e3 / st.matvecs,
+            "roofline": {"bound": "hbm", "algorithmic_bytes": nbytes, "achieved": nbytes / dt / 1e9,
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / dt / 1e9 / HBM_PEAK_GBS,
+                         "definition": "reference CGS2 semantics over the recorded schedule"}}
 
 
 def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
@@ -285,6 +366,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-yield and gradient probes")
     ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
+    ap.add_argument("--operator", default="production", choices=("production", "random"),
+                    help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol"),
@@ -307,7 +390,8 @@ def main() -> int:
     h = core.build_hamiltonian(90.0, 0.05)
     assert h.uses_pair_stencil
     if args.mode == "propagate":
-        return propagate_probe(h, args)
+        print(json.dumps(propagate_probe(h, args)), flush=True)
+        return 0
     if args.mode == "batch":
         print(json.dumps(batch_probe(h, args)), flush=True)
         return 0
@@ -320,6 +404,8 @@ def main() -> int:
     if args.mode == "sobol":
         print(json.dumps(sobol_probe(args)), flush=True)
         return 0
+    if args.operator == "random":
+        h = workloads.full_random_system(seed=1).hamiltonian()
     h._device_op().set_kernel(args.kernel)
     n = h.dimension
     B = args.batch
@@ -369,20 +455,50 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "c128",
         "data": "synthetic",
-        "config": {"workload": "batched H·ψ, production FAD-TrpH operator (theta=90 deg, 50 uT, "
-                               "EH D/J=-3, Q on), dim 221184, complex128",
+        "config": {"workload": ("batched H·ψ, production FAD-TrpH operator (theta=90 deg, 50 uT, "
+                                "EH D/J=-3, Q on), dim 221184, complex128") if args.operator == "production" else
+                               ("batched H·ψ, 14-nucleus random-tensor system (seed 1), dim 221184, complex128"),
                    "batch_vectors": B, "dimension": n,
                    "kernel": {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 9: "k_local_apply"}[h._device_op().kernel]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(B)},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(B) if args.operator == "production" else None},
     }
+    if not args.no_extras:
+        # the same kernel at other batch sizes: B <= 32 fits the 256 MB Infinity Cache, B >= 128 is HBM-bound
+        del y
+        sweep = {}
+        for bs in (16, 64, 1024):
+            if bs <= B:
+                xs = x[:bs]
+            else:   # synthetic data drawn on the device (the values do not affect the timing)
+                gen = torch.Generator(device="cuda")
+                gen.manual_seed(1)
+                xs = torch.randn((bs, n), dtype=torch.complex128, device="cuda", generator=gen)
+            ys = torch.empty_like(xs)
+            for _ in range(3):
+                h.apply_device(xs, ys, batch=bs, ld=n)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 20
+            e0.record()
+            for _ in range(reps):
+                h.apply_device(xs, ys, batch=bs, ld=n)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbs = BYTES_PER_AMPLITUDE * n * bs / (ms / 1e3) / 1e9
+            sweep[str(bs)] = {"ms_per_launch": ms, "hpsi_per_s": bs / (ms / 1e3), "achieved_GBps": gbs,
+                              "frac": gbs / HBM_PEAK_GBS}
+            del xs, ys
+        result["hpsi_sweep"] = sweep
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
     if not args.no_extras:
         # the other two parts of the metric, from short probes of the same production operator
         h._device_op().set_kernel(0)
-        del x, y
+        del x
         torch.cuda.empty_cache()
         print("[bench] batched-trajectory probe", file=sys.stderr, flush=True)
         result["batched"] = batch_probe(h, args, B=32, horizon=0.25, plain_sample=4)
@@ -392,6 +508,15 @@ def main() -> int:
         result["spectral"] = spectral_probe(args, with_cpu=not args.no_cpu)
         print("[bench] Sobol reaction-time probe", file=sys.stderr, flush=True)
         result["sobol"] = sobol_probe(args)
+        print("[bench] propagation probe", file=sys.stderr, flush=True)
+        result["propagation"] = propagate_probe(h, args, horizon=0.1)
+    if not args.no_cpu and not args.no_extras:
+        print("[bench] CPU trajectory and all-core baselines", file=sys.stderr, flush=True)
+        extra = cpu_extra_baselines()
+        result["cpu_baseline_extra"] = extra
+        if "propagation" in result:
+            result["propagation"]["gpu_over_cpu_core"] = extra["prefix_seconds"] / result["propagation"]["seconds"]
+        result["gpu_over_cpu_all_core"] = value / extra["all_core_hpsi_per_s"]
     print(json.dumps(result), flush=True)
     return 0
 
