@@ -410,6 +410,43 @@ def yield_and_gradient(base, hamiltonian_for, propagator, initial_states, times_
     return YieldGradient(yields, fgrad, pgrad, fy._saved["pops"], rec)
 
 
+@dataclass
+class PulseAscentResult:
+    amplitudes_uT: np.ndarray      # (S,) final pulse amplitudes
+    mean_yields: np.ndarray        # (steps + 1,) ensemble-mean yield before each step and at the end
+    gradients: np.ndarray          # (steps, S) ensemble-mean dPhi/da per uT at each step
+
+
+def pulse_ascent(problem: FieldYield, theta_deg: float, phi_deg: float, field_uT: float, amplitudes_uT, *,
+                 steps: int = 50, learning_rate: float = 1.0, bound_uT=None, callback=None) -> PulseAscentResult:
+    """Gradient ascent of the ensemble-mean singlet yield over a piecewise-constant pulse.
+
+    One amplitude vector (uT, one value per sub-interval of ``problem``'s pulse window) is shared by
+    every random-phase state of ``problem``. Each step is one batched forward run and one adjoint
+    sweep; the amplitudes move by ``learning_rate`` x the ensemble-mean gradient, clipped to
+    +-``bound_uT`` when given. ``callback(step, mean_yield, amplitudes, gradient)`` runs after each step.
+    """
+    if problem.pulse_window is None:
+        raise ValueError("pulse ascent needs a FieldYield with pulse_window_us")
+    a = np.array(amplitudes_uT, dtype=float).reshape(-1)
+    nb = int(problem.initial_states.shape[0])
+    th, ph, fb = [float(theta_deg)] * nb, [float(phi_deg)] * nb, [float(field_uT)] * nb
+    means, grads = [], []
+    for step in range(int(steps)):
+        y = problem.forward(th, ph, fb, np.tile(a, (nb, 1)))
+        _, pgrad, _ = problem.backward()
+        g = pgrad.mean(axis=0)
+        means.append(float(y.mean()))
+        grads.append(g)
+        a = a + learning_rate * g
+        if bound_uT is not None:
+            a = np.clip(a, -float(bound_uT), float(bound_uT))
+        if callback is not None:
+            callback(step, means[-1], a, g)
+    means.append(float(problem.forward(th, ph, fb, np.tile(a, (nb, 1))).mean()))
+    return PulseAscentResult(a, np.array(means), np.array(grads).reshape(len(grads), a.size))
+
+
 def singlet_yield(problem: FieldYield, field_params, pulse_amplitudes=None):
     """Differentiable yields: ``field_params`` (B, 3) float64 torch tensor of (theta [rad], phi [rad],
     |B| [uT]); ``pulse_amplitudes`` optional (B, S) tensor [uT].  Returns (B,) yields whose

@@ -439,3 +439,37 @@ class BatchedLanczosPropagator:
             values, bounds, errors, stats, record = results[b]
             out.append(PropagationResult(traj_times[b], values, bounds, errors, None, None, stats, record, states[b]))
         return out
+
+
+def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, phis_deg, fields_uT, times_us,
+                       lifetime_us, *, batch_size: int = 32, progress_callback=None) -> np.ndarray:
+    """Singlet yields of every (random-phase state, field direction, field strength): (S, D, F).
+
+    The directional-map workload: many random-phase vectors x field directions (e.g.
+    ``workloads.fibonacci_sphere``) x field strengths, run ``batch_size`` trajectories per batched
+    propagation. ``hamiltonian_for(theta_deg, field_mT, phi_deg)`` builds each trajectory's
+    Hamiltonian; only its electron part may differ from ``base``. The yield of each trajectory is the
+    reference's Simpson yield on ``times_us`` (R/rpm_quantum_needle_jd_sweeps.py:418-420). The map
+    is ``yields.mean(axis=0)``.
+    """
+    from .quadrature import integrate_yield
+
+    states = np.asarray(states) if not nat.is_torch(states) else states
+    thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
+    phis = np.broadcast_to(np.asarray(phis_deg, dtype=float), thetas.shape)
+    fields = np.atleast_1d(np.asarray(fields_uT, dtype=float))
+    S, D, F = states.shape[0], thetas.size, fields.size
+    combos = [(s, d, f) for s in range(S) for d in range(D) for f in range(F)]
+    out = np.empty((S, D, F))
+    for c0 in range(0, len(combos), max(1, int(batch_size))):
+        chunk = combos[c0:c0 + max(1, int(batch_size))]
+        hs = [hamiltonian_for(float(thetas[d]), 1e-3 * float(fields[f]), float(phis[d])) for _, d, f in chunk]
+        pairs = pair_matrices(base, hs)
+        idx = [s for s, _, _ in chunk]
+        x0 = states[idx] if not nat.is_torch(states) else states[idx]
+        results = propagator.propagate(x0, times_us, pairs=pairs)
+        for (s, d, f), r in zip(chunk, results):
+            out[s, d, f] = integrate_yield(r.times_us, r.observable_values, lifetime_us)
+        if progress_callback is not None:
+            progress_callback(min(c0 + len(chunk), len(combos)), len(combos))
+    return out

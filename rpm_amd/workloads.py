@@ -95,3 +95,18 @@ def normal_vectors(n: int, batch: int, seed: int = 0) -> np.ndarray:
     v = rng.normal(size=(batch, n)) + 1j * rng.normal(size=(batch, n))
     v /= np.linalg.norm(v, axis=1, keepdims=True)
     return v
+
+
+def fibonacci_sphere(n: int):
+    """n field directions spread evenly over the sphere (golden-angle spiral): (theta_deg, phi_deg).
+
+    Point i sits at z = 1 - (2i + 1) / n and azimuth i times the golden angle, so the points cover equal
+    areas of the sphere. These are the orientations of the directional-map workload (64 of them).
+    """
+    if n < 1:
+        raise ValueError("need at least one direction")
+    i = np.arange(n, dtype=float) + 0.5
+    z = 1.0 - 2.0 * i / n
+    theta = np.degrees(np.arccos(z))
+    phi = np.degrees(np.mod(np.pi * (1.0 + math.sqrt(5.0)) * i, 2.0 * np.pi))
+    return theta, phi

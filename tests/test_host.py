@@ -242,3 +242,19 @@ def test_seeds_and_initial_state_digests_match_archive():
         s = pcore.random_phase_initial_state(model.SYSTEM_DIMENSIONS, pcore.MASTER_SEED, sid)
         assert pcore.initial_state_sha256(s) == digest
     assert pcore.sample_seed(5, 7) == ocore.seed_for(5, 7)
+
+
+def test_fibonacci_sphere_directions():
+    from rpm_amd import workloads
+    for n in (1, 7, 64):
+        th, ph = workloads.fibonacci_sphere(n)
+        assert th.shape == ph.shape == (n,)
+        z = np.cos(np.radians(th))
+        np.testing.assert_allclose(z, 1.0 - (2.0 * np.arange(n) + 1.0) / n, atol=1e-12)   # equal-area bands
+        assert np.all((ph >= 0.0) & (ph < 360.0))
+    th, ph = workloads.fibonacci_sphere(64)
+    v = np.stack([np.sin(np.radians(th)) * np.cos(np.radians(ph)), np.sin(np.radians(th)) * np.sin(np.radians(ph)),
+                  np.cos(np.radians(th))], axis=1)
+    assert np.abs(v.mean(axis=0)).max() < 0.01          # no preferred direction
+    with pytest.raises(ValueError):
+        workloads.fibonacci_sphere(0)
