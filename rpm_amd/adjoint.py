@@ -32,7 +32,6 @@ from __future__ import annotations
 import ctypes
 import math
 from dataclasses import dataclass
-from typing import Sequence
 
 import numpy as np
 
