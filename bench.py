@@ -324,29 +324,30 @@ def spectral_probe(args, with_cpu=True) -> dict:
     out = {"mode": "spectral", "orientations": len(rows), "lifetimes": len(model.DEFAULT_LIFETIMES_US),
            "seconds": dt, "orientations_per_s": len(rows) / dt,
            "max_initial_singlet_error": max(abs(r["initial_singlet"] - 1.0) for r in rows)}
-    # the line-binning kernel alone: one launch = 32 FAD orientations x 6 pairs, grid stride of the curve
+    # the line-binning kernel alone: one launch = 32 TrpH orientations x 6 pairs (TrpH, d = 384, is the
+    # largest block the curve bins: FAD goes as 288 + 2 x 144), grid stride of the curve
     fad, trp = spectral.reference_models(0.0)
-    sa = spectral.diagonalize_radical(fad, thetas[:32], 50.0)
+    sa = [spectral.diagonalize_radical(b, thetas[:32], 50.0) for b in spectral.symmetry_blocks(fad)]
     sb = spectral.diagonalize_radical(trp, thetas[:32], 50.0)
     delta = (1.0 / max(model.DEFAULT_LIFETIMES_US)) / 8.0
-    ha = spectral.half_width_bins(float(sa.max_gap.max()), delta)
+    ha = spectral.half_width_bins(float(spectral._max_gap(sa).max()), delta)
     hb = spectral.half_width_bins(float(sb.max_gap.max()), delta)
     stride = spectral._fast_length(2 * ha + 2 * hb + 1)
     buf = torch.empty((32, len(spectral.PAIRS), stride), dtype=torch.complex128, device="cuda")
     flag = torch.zeros(1, dtype=torch.int32, device="cuda")
-    spectral.binned_spectra(sa, delta, ha, stride, out=buf, overflow=flag)
+    spectral.binned_spectra(sb, delta, hb, stride, out=buf, overflow=flag)
     reps = 10
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    start.record()
+    start.record(torch.cuda.current_stream())
     for _ in range(reps):
-        spectral.binned_spectra(sa, delta, ha, stride, out=buf, overflow=flag)
-    stop.record()
+        spectral.binned_spectra(sb, delta, hb, stride, out=buf, overflow=flag)
+    stop.record(torch.cuda.current_stream())
     torch.cuda.synchronize()
     ms = start.elapsed_time(stop) / reps
-    d = fad.dimension
+    d = trp.dimension
     nbytes = buf.numel() * 16 + 32 * 3 * d * d * 16 + 32 * d * 8   # grid written once + T_x,y,z + eigenvalues read
-    out["bin_kernel"] = {"kernel": "k_bin_tiles", "orientations": 32, "radical": "FAD (d=576)", "grid_stride": stride,
+    out["bin_kernel"] = {"kernel": "k_bin_tiles", "orientations": 32, "radical": "TrpH (d=384)", "grid_stride": stride,
                          "ms_per_launch": ms, "bytes_per_launch": nbytes, "achieved_GBps": nbytes / ms / 1e6,
                          "frac_of_hbm_peak": nbytes / ms / 1e6 / HBM_PEAK_GBS}
     del buf, sa, sb

@@ -226,16 +226,20 @@ int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro
 
 /*
  * Factorized spectral yields (R/rpm_quantum_needle_reproduction.py:145-172, transition_lines and
- * binned_spectrum): the transition lines of one radical, frequency lambda_i - lambda_j and weight
- * Tp[i][j] conj(Tq[i][j]) / nuclear_dimension, deposited by linear interpolation at grid index
- * (lambda_i - lambda_j) / delta + half, for the six pairs (p, q) = (0,0) (1,1) (2,2) (0,1) (0,2) (1,2).
+ * binned_spectrum): the transition lines of one radical (or of one symmetry block of it), frequency
+ * lambda_i - lambda_j and weight Tp[i][j] conj(Tq[i][j]) / weight_divisor, deposited by linear
+ * interpolation at grid index (lambda_i - lambda_j) / delta + half, for the six pairs
+ * (p, q) = (0,0) (1,1) (2,2) (0,1) (0,2) (1,2). weight_divisor is the radical's nuclear dimension,
+ * divided by the block's multiplicity when a block stands for several identical ones.
  * d_eigenvalues: norient x d, ascending per orientation. d_ops: norient x 3 x d x d complex (T_x, T_y,
- * T_z in the eigenbasis, row-major, Hermitian). d_grid: norient x 6 x grid_stride complex, written in
- * full: bins 0 .. 2 half hold the spectrum, the rest zeros. If a line falls outside bins 0 .. 2 half,
+ * T_z in the eigenbasis, row-major, Hermitian). d_grid: norient x 6 x grid_stride complex: with
+ * accumulate = 0 written in full (bins 0 .. 2 half hold the spectrum, the rest zeros); with
+ * accumulate = 1 the lines are added to what it holds. If a line falls outside bins 0 .. 2 half,
  * *d_overflow (device int) is incremented.
  */
-int rpm_bin_lines(const double* d_eigenvalues, const void* d_ops, int d, int norient, int nuclear_dimension,
-                  double delta, long long half, void* d_grid, long long grid_stride, int* d_overflow, void* stream);
+int rpm_bin_lines(const double* d_eigenvalues, const void* d_ops, int d, int norient, double weight_divisor,
+                  double delta, long long half, void* d_grid, long long grid_stride, int accumulate, int* d_overflow,
+                  void* stream);
 
 /*
  * Yields of the convolved spectra by Parseval (R/rpm_quantum_needle_reproduction.py:175-214, the FFT

@@ -75,8 +75,8 @@ SIGNATURES = {
     "rpm_eigh_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _sz, ctypes.c_int, _vp]),
     "rpm_spectral_yields": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, _ll, ctypes.c_int, ctypes.c_int, _vp,
                                            _vp]),
-    "rpm_bin_lines": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ll,
-                                     _vp, _ll, _vp, _vp]),
+    "rpm_bin_lines": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ll,
+                                     _vp, _ll, ctypes.c_int, _vp, _vp]),
     "rpm_adjoint_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
     "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,

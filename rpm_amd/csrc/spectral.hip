@@ -93,7 +93,8 @@ template <bool kFlat>
 __global__ void __launch_bounds__(kBinThreads) k_bin_tiles(const double* __restrict__ lam_all,
                                                            const c128* __restrict__ ops, int d, double m,
                                                            double delta, long long half, c128* __restrict__ grid,
-                                                           long long gstride, int* __restrict__ overflow) {
+                                                           long long gstride, int* __restrict__ overflow,
+                                                           int accumulate) {
   __shared__ double t_re[kTileBins];
   __shared__ double t_im[kTileBins];
   __shared__ double s_lam[kFlat ? kFlatRows : 1];
@@ -120,11 +121,12 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_tiles(const double* __restr
     if (!(lo_pos >= 0.0) || !(hi_pos + 1.0 < static_cast<double>(size))) atomicAdd(overflow, 1);
   }
   if (b0 >= size) {
-    // padding of the FFT input: zeros (block-uniform branch)
-    for (int k = threadIdx.x; k < kTileBins; k += kBinThreads) {
-      const long long idx = b0 + k;
-      if (idx < gstride) g[idx] = c128{0.0, 0.0};
-    }
+    // padding of the FFT input: zeros (block-uniform branch); nothing to add when accumulating
+    if (!accumulate)
+      for (int k = threadIdx.x; k < kTileBins; k += kBinThreads) {
+        const long long idx = b0 + k;
+        if (idx < gstride) g[idx] = c128{0.0, 0.0};
+      }
     return;
   }
 
@@ -217,7 +219,14 @@ __global__ void __launch_bounds__(kBinThreads) k_bin_tiles(const double* __restr
 
   for (int k = threadIdx.x; k < kTileBins; k += kBinThreads) {
     const long long idx = b0 + k;
-    if (idx < gstride) g[idx] = c128{t_re[k], t_im[k]};
+    if (idx < gstride) {
+      if (accumulate) {
+        const c128 old = g[idx];
+        g[idx] = c128{old.re + t_re[k], old.im + t_im[k]};
+      } else {
+        g[idx] = c128{t_re[k], t_im[k]};
+      }
+    }
   }
 }
 
@@ -300,23 +309,24 @@ __global__ void __launch_bounds__(kYieldThreads) k_spectral_yields(const c128* _
 
 using namespace rpm;
 
-extern "C" int rpm_bin_lines(const double* d_eigenvalues, const void* d_ops, int d, int norient,
-                             int nuclear_dimension, double delta, long long half, void* d_grid, long long grid_stride,
+extern "C" int rpm_bin_lines(const double* d_eigenvalues, const void* d_ops, int d, int norient, double weight_divisor,
+                             double delta, long long half, void* d_grid, long long grid_stride, int accumulate,
                              int* d_overflow, void* stream) {
   if (!d_eigenvalues || !d_ops || !d_grid || !d_overflow || d < 1 || d > 46340 || norient < 1 || norient > 65535 ||
-      nuclear_dimension < 1 || !(delta > 0.0) || half < 1 || grid_stride < 2 * half + 1)
+      !(weight_divisor > 0.0) || !(delta > 0.0) || half < 1 || grid_stride < 2 * half + 1 ||
+      (accumulate != 0 && accumulate != 1))
     return fail(RPM_E_INVALID, "rpm_bin_lines: bad arguments");
   const long long tiles = (grid_stride + kTileBins - 1) / kTileBins;
   if (tiles > 0x7fffffffLL) return fail(RPM_E_INVALID, "rpm_bin_lines: grid too long");
   const dim3 grid(static_cast<unsigned>(tiles), norient, kSpectralPairs);
   if (d <= kFlatRows)
     hipLaunchKernelGGL(k_bin_tiles<true>, grid, dim3(kBinThreads), 0, static_cast<hipStream_t>(stream), d_eigenvalues,
-                       static_cast<const c128*>(d_ops), d, static_cast<double>(nuclear_dimension), delta, half,
-                       static_cast<c128*>(d_grid), grid_stride, d_overflow);
+                       static_cast<const c128*>(d_ops), d, weight_divisor, delta, half,
+                       static_cast<c128*>(d_grid), grid_stride, d_overflow, accumulate);
   else
     hipLaunchKernelGGL(k_bin_tiles<false>, grid, dim3(kBinThreads), 0, static_cast<hipStream_t>(stream),
-                       d_eigenvalues, static_cast<const c128*>(d_ops), d, static_cast<double>(nuclear_dimension), delta,
-                       half, static_cast<c128*>(d_grid), grid_stride, d_overflow);
+                       d_eigenvalues, static_cast<const c128*>(d_ops), d, weight_divisor, delta,
+                       half, static_cast<c128*>(d_grid), grid_stride, d_overflow, accumulate);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

@@ -143,3 +143,16 @@ def test_many_lifetimes_and_phi():
     assert abs(row["initial_singlet"] - initial) < 1e-12
     for tau in taus:
         assert abs(row[f"phi_s_tau_{tau:g}_us"] - want[tau]) < 1e-12, tau
+
+
+@pytest.mark.parametrize("scale", [0.0, 1.0])
+def test_multiplet_blocks_match_the_whole_radical(scale):
+    """The block route (FAD as 288 + 2 x 144) against the whole 576-state radical, Q off and on."""
+    print(f"case: symmetry blocks, Q scale {scale}", flush=True)
+    thetas = [0.0, 37.0, 89.5, 90.0, 151.0]
+    blocks = ps.solve_orientations(thetas, 50.0, TAUS, 8.0, quadrupole_scale=scale, phis_deg=20.0)
+    whole = ps.solve_orientations(thetas, 50.0, TAUS, 8.0, quadrupole_scale=scale, phis_deg=20.0, symmetry=False)
+    for a, b in zip(blocks, whole):
+        assert a.keys() == b.keys()
+        for k in a:
+            assert abs(a[k] - b[k]) < 1e-12, (a["theta_deg"], k, a[k], b[k])

@@ -78,10 +78,12 @@ def test_spectral_entry_points_reject_bad_arguments_without_gpu():
     vp = ctypes.c_void_p
     one = vp(1)
     # rpm_bin_lines: null buffers, a stride shorter than the grid, a non-positive bin width
-    assert lib.rpm_bin_lines(None, one, 4, 1, 2, 0.1, 10, one, 21, one, None) == _native.RPM_E_INVALID
-    assert lib.rpm_bin_lines(one, one, 4, 1, 2, 0.1, 10, one, 20, one, None) == _native.RPM_E_INVALID
-    assert lib.rpm_bin_lines(one, one, 4, 1, 2, 0.0, 10, one, 21, one, None) == _native.RPM_E_INVALID
-    assert lib.rpm_bin_lines(one, one, 0, 1, 2, 0.1, 10, one, 21, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(None, one, 4, 1, 2.0, 0.1, 10, one, 21, 0, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 2.0, 0.1, 10, one, 20, 0, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 2.0, 0.0, 10, one, 21, 0, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 0, 1, 2.0, 0.1, 10, one, 21, 0, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 0.0, 0.1, 10, one, 21, 0, one, None) == _native.RPM_E_INVALID
+    assert lib.rpm_bin_lines(one, one, 4, 1, 2.0, 0.1, 10, one, 21, 2, one, None) == _native.RPM_E_INVALID
     # rpm_spectral_yields: more than 8 lifetimes per pass, empty batch
     assert lib.rpm_spectral_yields(one, one, one, one, 2, 100, 9, 4, one, None) == _native.RPM_E_INVALID
     assert lib.rpm_spectral_yields(one, one, one, one, 0, 100, 6, 4, one, None) == _native.RPM_E_INVALID
