@@ -44,7 +44,7 @@ from .model import (DEFAULT_LIFETIMES_US, FAD_NUCLEI, GAMMA_E_MHZ_PER_MT, NOMINA
 
 MARGIN_BINS = 3          # R/rpm_quantum_needle_reproduction.py:180
 PAIRS = ((0, 0), (1, 1), (2, 2), (0, 1), (0, 2), (1, 2))   # the grid order of rpm_bin_lines
-DEFAULT_CHUNK = 32       # orientations per pass; about 0.5 GB of device memory each at the Fig. 1B grid
+DEFAULT_CHUNK = 16       # orientations per grid pass; about 1 GB of device memory each at the Fig. 1B grid
 
 __all__ = ["RadicalModel", "Spectra", "build_radical", "reference_models", "diagonalize_radical",
            "symmetry_blocks", "eigh_batched", "binned_spectra", "correlation_yields", "solve_orientations", "solve_orientation",
@@ -454,7 +454,12 @@ def correlation_yields(spec_a, spec_b, delta_omega: float, lifetimes_us: Sequenc
     return _yields_on_grid(torch, spec_a, spec_b, g)
 
 
-SUPER_CHUNK = 512   # orientations whose spectra are held at once (about 23 MB each for FAD + TrpH)
+SUPER_CHUNK = 512   # orientations diagonalised at once and sharing one grid (about 12 MB each for FAD + TrpH)
+
+
+def _rows(spec: Spectra, part: slice) -> Spectra:
+    """The orientations `part` of a batch of spectra (contiguous views)."""
+    return Spectra(spec.eigenvalues[part], spec.ops[part], spec.nuclear_dimension, spec.multiplicity)
 
 
 def solve_orientations(thetas_deg, field_microT: float = 50.0,
@@ -485,24 +490,23 @@ def solve_orientations(thetas_deg, field_microT: float = 50.0,
     rows = []
     for s0 in range(0, len(thetas), SUPER_CHUNK):
         s1 = min(len(thetas), s0 + SUPER_CHUNK)
-        spectra = []
-        for c0 in range(s0, s1, step):
-            sl = slice(c0, min(s1, c0 + step))
-            spectra.append((sl, [diagonalize_radical(m, thetas[sl], field_microT, phis[sl], eigh=eigh) for m in parts_a],
-                            [diagonalize_radical(m, thetas[sl], field_microT, phis[sl], eigh=eigh) for m in parts_b]))
-        gap_a = max(float(_max_gap(sa).max()) for _, sa, _ in spectra)
-        gap_b = max(float(_max_gap(sb).max()) for _, _, sb in spectra)
-        g = _grid(torch, spectra[0][1][0].eigenvalues.device, gap_a, gap_b, delta, lifetimes)
-        while spectra:
-            sl, sa, sb = spectra.pop(0)
-            initial, yields = _yields_on_grid(torch, sa, sb, g)
-            del sa, sb
-            for i, theta in enumerate(thetas[sl]):
+        # eigen-decompositions of the whole super-chunk at once (about 12 MB per FAD + TrpH orientation):
+        # rocSOLVER's batched kernels are launch-bound at small batches
+        sa = [diagonalize_radical(m, thetas[s0:s1], field_microT, phis[s0:s1], eigh=eigh) for m in parts_a]
+        sb = [diagonalize_radical(m, thetas[s0:s1], field_microT, phis[s0:s1], eigh=eigh) for m in parts_b]
+        g = _grid(torch, sa[0].eigenvalues.device, float(_max_gap(sa).max()), float(_max_gap(sb).max()), delta,
+                  lifetimes)
+        # the frequency grids (about 0.4 GB per orientation at the Fig. 1B grid) are formed `chunk` at a time
+        for c0 in range(0, s1 - s0, step):
+            part = slice(c0, min(s1 - s0, c0 + step))
+            initial, yields = _yields_on_grid(torch, [_rows(x, part) for x in sa], [_rows(x, part) for x in sb], g)
+            for i, theta in enumerate(thetas[s0 + part.start:s0 + part.stop]):
                 row = {"theta_deg": float(theta), "initial_singlet": float(initial[i]),
                        "delta_omega_rad_per_us": delta}
                 for j, tau in enumerate(lifetimes):
                     row[f"phi_s_tau_{tau:g}_us"] = float(yields[i, j])
                 rows.append(row)
+        del sa, sb
     return rows
 
 
