@@ -4,10 +4,11 @@
 //
 // Orthogonalisation of w = H v_j (two projections, as classical Gram-Schmidt
 // applied twice): the three-term step w - beta_{j-1} v_{j-1} - alpha_j v_j
-// (beta subtracted inside the matvec, alpha_j = <v_j, H v_j> from the matvec's
-// partial sums) and then one full pass w <- w - V (V^H w).  That is two reads
-// of V per vector.  The reference subtracts the same three terms and then
-// projects twice more; the bases agree to round-off.
+// (beta subtracted inside the matvec) and then one full projection
+// w <- w - V (V^H w), applied one step late so that the projection of step j
+// and the materialisation of v_{j+1} share one read of V (k_lcgs_pass).  The
+// reference subtracts the same three terms and then projects twice more; the
+// bases agree to round-off.
 //
 // Follows R/rpm_statevector_engine.py:651-688 (basis), 492-497 (singlet
 // population), 521-529 (projected observable), 904 (endpoint update).
@@ -97,9 +98,10 @@ __device__ inline bool basis_live(const BasisCtl* ctl, int b, int j, int need) {
   return c.active && j + need < c.target;
 }
 
-// Norm of each starting state; resets the control blocks.
+// Norm of each starting state; resets the control blocks and the k_lcgs_finish counters.
 __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restrict__ partial, int nblk,
                                                           double* __restrict__ norm_out, BasisCtl* ctl,
+                                                          unsigned* __restrict__ counter,
                                                           const int* __restrict__ targets, int uniform, long long n) {
   __shared__ double sm[kThreads / kWave];
   const int b = blockIdx.x;
@@ -110,6 +112,8 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
     const double nrm = std::sqrt(s);
     int count = targets ? min(targets[b], uniform) : uniform;
     if (count > n) count = static_cast<int>(n);
+    counter[b] = 0;
+    ctl[b].fix = 0;
     if (count <= 0) {
       // not part of this call: its norm, basis and tridiagonal stay as they are
       ctl[b].active = 0;
@@ -156,154 +160,233 @@ __device__ inline double alpha_sum(const c128* __restrict__ dots, int units, int
   return block_sum(s, sm).re;
 }
 
-// alpha[b*astride + j] = Re sum_u dots[b][u]. A single trajectory's basis forms alpha inside
-// k_gs_dots for all but its last step.
-__global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dots, int units, double* __restrict__ alpha,
-                                                    int astride, int j, const BasisCtl* ctl) {
+// atilde[b] = Re sum_u dots[b][u] (alpha~ of the lagged pass). A single trajectory's basis forms it
+// inside k_lcgs_pass instead.
+__global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dots, int units,
+                                                    double* __restrict__ atilde, int j, const BasisCtl* ctl) {
   const int b = blockIdx.x;
   if (!basis_live(ctl, b, j, 0)) return;
   __shared__ c128 sm[kThreads / kWave];
   const double a = alpha_sum(dots, units, b, sm);
-  if (threadIdx.x == 0) alpha[b * astride + j] = a;
+  if (threadIdx.x == 0) atilde[b] = a;
 }
 
-// w -= sum_c V_c h_c (one element per thread) and partial sums of |w|^2; with
-// alpha, the last column's coefficient is h_{k-1} + alpha_j (the three-term
-// step's alpha_j V_j subtracted together with the projection).
-constexpr int kUpdChunk = kThreads;
-__global__ void __launch_bounds__(kThreads) k_update(const c128* __restrict__ V, long long vstride, long long n, int k,
-                                                     const c128* __restrict__ h, int hstride,
-                                                     const double* __restrict__ alpha, int astride,
-                                                     c128* __restrict__ W, double* __restrict__ norm_partial, int j,
-                                                     const BasisCtl* ctl) {
-  const int b = blockIdx.y;
-  if (!basis_live(ctl, b, j, 1)) return;
-  __shared__ double sm[kThreads / kWave];
-  __shared__ c128 hs[kMaxBasis];
-  const c128* Vb = V + b * vstride;
-  if (threadIdx.x < k) {
-    c128 v = h[b * hstride + threadIdx.x];
-    if (alpha && threadIdx.x == k - 1) v.re += alpha[b * astride + j];
-    hs[threadIdx.x] = v;
-  }
-  __syncthreads();
-  const c128* hb = hs;
-  c128* w = W + b * n;
-  const long long i = static_cast<long long>(blockIdx.x) * kUpdChunk + threadIdx.x;
-  c128 wv = i < n ? w[i] : cmake(0.0, 0.0);
-  if (i < n) {
-    int c = 0;
-    for (; c + 4 <= k; c += 4) {
-      const c128 v0 = Vb[static_cast<long long>(c) * n + i], v1 = Vb[static_cast<long long>(c + 1) * n + i];
-      const c128 v2 = Vb[static_cast<long long>(c + 2) * n + i], v3 = Vb[static_cast<long long>(c + 3) * n + i];
-      wv = cfms(v0, hb[c], wv);
-      wv = cfms(v1, hb[c + 1], wv);
-      wv = cfms(v2, hb[c + 2], wv);
-      wv = cfms(v3, hb[c + 3], wv);
-    }
-    for (; c < k; ++c) wv = cfms(Vb[static_cast<long long>(c) * n + i], hb[c], wv);
-    w[i] = wv;
-  }
-  double s = fma(wv.re, wv.re, wv.im * wv.im);
-  s = block_sum(s, sm);
-  if (threadIdx.x == 0) norm_partial[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
-}
+// ---------------------------------------------------------------- lagged one-pass orthogonalisation
+//
+// Step j works on the materialised basis v_0 .. v_{j-1} and the vector x_j that the matvec
+// consumed, which is v_j plus a pending O(eps) combination of the basis, v_j = x_j - V_<j c_j
+// (c_j from step j-1). With u = H x_j - beta_{j-1} v_{j-1} (the matvec), one pass over V_<j
+//   * materialises v_j = x_j - V_<j c_j in V[j],
+//   * forms D = V_<j^H u, gamma = v_j^H u and y = u - alpha~ v_j (alpha~ = x_j^H H x_j from the
+//     matvec's partials), with |y|^2.
+// Then (k_lcgs_finish), with H V_<j = V_<=j T + O(eps):
+//   alpha_j = Re gamma - beta_{j-1} Re c_j[j-1],
+//   w''_j = (1 - V_<=j V_<=j^H) u = y - V_<=j g, g = [D; gamma - alpha~],
+//   beta_j^2 = |y|^2 - |g|^2, x_{j+1} = y / beta_j, c_{j+1} = g / beta_j.
+// w''_j is the three-term step followed by one full projection, as in two-pass classical
+// Gram-Schmidt; the projection is only applied one step later, inside the next pass. The matvec
+// acting on x_{j+1} instead of v_{j+1} adds H V c, which lies in the span of the basis and is
+// removed by the next projection; |c| grows by about |H| / beta per step, so when |c| passes
+// kFixThreshold, or when |g| is not small against |y| (near breakdown), k_lcgs_finish applies the
+// correction to y explicitly and takes beta_j from the corrected vector. One read of V per step
+// instead of two.
+constexpr int kPassPer = 4;                // 64-element runs per k_lcgs_pass block (2 and 8 measured slower)
+constexpr int kPassElems = kPassPer * kWave;
+constexpr double kFixThreshold = 1e-6;     // |c_{j+1}| above which the correction is applied at once
+constexpr double kCancel = 1e-2;           // beta^2 <= kCancel |y|^2: the norm is taken explicitly
+constexpr int kFixBlocks = 256;            // blocks per trajectory of k_lcgs_finish
+constexpr int kPassWaves = 4;
+static_assert(kPassWaves * kWave == kThreads, "k_lcgs_pass shares the kThreads-wide reductions");
 
-// Fused pass over the basis: partial[c][blk] = sum_x conj(V_c[x]) w'[x] for all
-// c < k with w' = w - alpha_j V_j (j = k-1), the finished three-term step.  w'
-// is formed in registers only (every wave reads w); k_update subtracts
-// alpha_j V_j together with the projection. With `adots`, every block first forms
-// alpha_j from the matvec's partials (block 0 stores it) instead of reading it.
-// Wave q of the 4-wave block owns columns c = q, q+4, ..., one element per lane
-// per iteration, so every lane keeps up to 16 column loads in flight and no LDS
-// staging is needed; each block covers `per` consecutive 64-element runs and
-// ends with one wave reduction per column.  k <= kMaxBasis.
-constexpr int kDotWaves = 4;
-constexpr int kDotCols = kMaxBasis / kDotWaves;
-// 64-element runs per k_gs_dots block. The same in batches, so a batched basis is bitwise the
-// single-trajectory basis.
-constexpr int kDotPer = 8;
-constexpr int kBatchCpw = 8;    // columns per wave of k_gs_dots in a batch (2 column groups at k = 64)
-static_assert(kDotWaves * kWave == kThreads, "k_gs_dots shares the kThreads-wide reductions");
-// CPW columns per wave: block z covers columns [z * 4 CPW, (z + 1) * 4 CPW), column c in wave c % 4.
-// Whatever CPW, a column's partial is formed by the same lanes over the same elements in the same
-// order, so the result is bitwise the same; a small CPW only spreads one vector over more blocks.
+// Columns c < j are dealt to the waves: column c to wave c % 4, CPW per wave. Whatever CPW, a
+// column's partial is formed by the same lanes over the same elements in the same order, and the
+// per-element correction is summed over the waves in a fixed order, so the basis is bitwise the
+// same for every CPW and batch size.
 template <int CPW>
-__global__ void __launch_bounds__(kDotWaves * kWave) k_gs_dots(const c128* __restrict__ V, long long vstride, long long n,
-                                                               int k, int per, double* __restrict__ alpha,
-                                                               int astride, const c128* __restrict__ adots, int units,
-                                                               const c128* __restrict__ W,
-                                                               c128* __restrict__ partial, long long pstride, int j,
-                                                               int need, const BasisCtl* ctl) {
+__global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, long long vstride, long long n, int j,
+                                                        const c128* __restrict__ cpend, int cstride,
+                                                        double* __restrict__ atilde,
+                                                        const c128* __restrict__ adots, int units, c128* __restrict__ U,
+                                                        c128* __restrict__ partial, long long pstride,
+                                                        double* __restrict__ ypart, const BasisCtl* ctl) {
   const int b = blockIdx.y;
-  if (!basis_live(ctl, b, j, adots ? 0 : need)) return;
+  if (!basis_live(ctl, b, j, 0)) return;
+  __shared__ c128 s_u[kPassElems];
+  __shared__ c128 s_corr[kPassWaves][kPassElems];
+  __shared__ c128 s_c[kMaxBasis];
   __shared__ c128 sm[kThreads / kWave];
-  double a;
+  __shared__ double smd[kThreads / kWave];
+  double at;
   if (adots) {
-    // alpha_j from the matvec's partials: every block forms the same sum in the same order as
-    // k_alpha (the partials come from the previous kernel, so no cross-block synchronisation is
-    // needed). Block 0 stores it, also when step j is the trajectory's last and the projection
-    // below is skipped.
-    a = alpha_sum(adots, units, b, sm);
-    if (blockIdx.x == 0 && blockIdx.z == 0 && threadIdx.x == 0) alpha[b * astride + j] = a;
-    if (!basis_live(ctl, b, j, need)) return;   // uniform over the blocks of vector b
+    // alpha~ from the matvec's partials, formed by every block in the same order as k_alpha;
+    // block 0 stores it for k_lcgs_finish
+    at = alpha_sum(adots, units, b, sm);
+    if (blockIdx.x == 0 && threadIdx.x == 0) atilde[b] = at;
   } else {
-    a = alpha[b * astride + j];
+    at = atilde[b];
   }
-  const c128* Vb = V + b * vstride;
-  const c128* w = W + b * n;
+  c128* Vb = V + b * vstride;
+  c128* u = U + b * n;
+  const long long x0 = static_cast<long long>(blockIdx.x) * kPassElems;
+  const int cnt = static_cast<int>(n - x0 < kPassElems ? n - x0 : kPassElems);
+  for (int e = threadIdx.x; e < cnt; e += kThreads) s_u[e] = u[x0 + e];
+  if (threadIdx.x < j) s_c[threadIdx.x] = cpend[b * cstride + threadIdx.x];
+  __syncthreads();
   const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
-  const c128* vj = Vb + static_cast<long long>(k - 1) * n;
-  const int c0 = static_cast<int>(blockIdx.z) * kDotWaves * CPW + q;
   c128 acc[CPW];
 #pragma unroll
   for (int i = 0; i < CPW; ++i) acc[i] = cmake(0.0, 0.0);
-  const long long x0 = static_cast<long long>(blockIdx.x) * per * kWave + lane;
-  for (int it = 0; it < per; ++it) {
-    const long long x = x0 + static_cast<long long>(it) * kWave;
-    if (x >= n) break;
-    c128 wv = w[x];
-    {
-      const c128 v = vj[x];
-      wv.re = fma(-a, v.re, wv.re);
-      wv.im = fma(-a, v.im, wv.im);
-    }
+  for (int it = 0; it < kPassPer; ++it) {
+    const int e = it * kWave + lane;
+    c128 corr = cmake(0.0, 0.0);
+    if (e < cnt) {
+      const c128 uu = s_u[e];
 #pragma unroll
-    for (int i = 0; i < CPW; ++i) {
-      const int c = c0 + kDotWaves * i;
-      if (c < k) acc[i] = cfmaconj(Vb[static_cast<long long>(c) * n + x], wv, acc[i]);
+      for (int i = 0; i < CPW; ++i) {
+        const int c = q + kPassWaves * i;
+        if (c < j) {
+          const c128 v = Vb[static_cast<long long>(c) * n + x0 + e];
+          acc[i] = cfmaconj(v, uu, acc[i]);
+          corr = cfma(v, s_c[c], corr);
+        }
+      }
     }
+    s_corr[q][e] = corr;
   }
   const int nblk = gridDim.x;
 #pragma unroll
   for (int i = 0; i < CPW; ++i) {
-    const int c = c0 + kDotWaves * i;
-    if (c < k) {
+    const int c = q + kPassWaves * i;
+    if (c < j) {
       const c128 t = wave_sum(acc[i]);
       if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
     }
   }
+  __syncthreads();
+  c128* vj = Vb + static_cast<long long>(j) * n;
+  c128 g = cmake(0.0, 0.0);
+  double yy = 0.0;
+  for (int e = threadIdx.x; e < cnt; e += kThreads) {
+    const c128 corr = cadd(cadd(cadd(s_corr[0][e], s_corr[1][e]), s_corr[2][e]), s_corr[3][e]);
+    const c128 v = csub(vj[x0 + e], corr);
+    vj[x0 + e] = v;
+    const c128 uu = s_u[e];
+    const c128 y = cmake(fma(-at, v.re, uu.re), fma(-at, v.im, uu.im));
+    u[x0 + e] = y;
+    g = cfmaconj(v, uu, g);
+    yy = fma(y.re, y.re, fma(y.im, y.im, yy));
+  }
+  g = block_sum(g, sm);
+  yy = block_sum(yy, smd);
+  if (threadIdx.x == 0) {
+    partial[b * pstride + static_cast<long long>(j) * nblk + blockIdx.x] = g;
+    ypart[static_cast<long long>(b) * nblk + blockIdx.x] = yy;
+  }
 }
 
-// beta_j = ||w||; breakdown test; scale for the next matvec.
-__global__ void __launch_bounds__(kThreads) k_finish_beta(const double* __restrict__ partial, int nblk,
-                                                          double* __restrict__ beta, int bstride, int j, double tol,
-                                                          BasisCtl* ctl) {
-  const int b = blockIdx.x;
-  if (!basis_live(ctl, b, j, 1)) return;
+// alpha_j, beta_j and c_{j+1} from the pass's sums h = [D; gamma] and |y|^2 (see above), and the
+// explicit correction when it is due. Grid (kFixBlocks, batch): every block forms the same scalars
+// in the same order; block 0 stores them. When the correction is due (|c_{j+1}| > kFixThreshold,
+// or |g| not small against |y|), every block subtracts its share of V_<=j g from y, and the last
+// block of the trajectory to finish (device-scope counter; the fences run on this rare path only)
+// takes beta_j from the corrected vector and clears c_{j+1}.
+__global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict__ V, long long vstride, long long n,
+                                                          const double* __restrict__ ypart, int nblk,
+                                                          const c128* __restrict__ h, int hstride,
+                                                          c128* __restrict__ cpend, int cstride,
+                                                          const double* __restrict__ atilde, double* __restrict__ alpha,
+                                                          double* __restrict__ beta, int astride, int j, double tol,
+                                                          c128* __restrict__ U, double* __restrict__ fpart,
+                                                          unsigned* __restrict__ counter, BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  if (!basis_live(ctl, b, j, 0)) return;
   __shared__ double sm[kThreads / kWave];
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nblk; i += kThreads) s += partial[static_cast<long long>(b) * nblk + i];
-  s = block_sum(s, sm);
-  if (threadIdx.x == 0) {
-    const double bj = std::sqrt(s);
-    beta[b * bstride + j] = bj;
+  __shared__ c128 gs[kMaxBasis];
+  __shared__ int is_last;
+  const int t = threadIdx.x;
+  const c128* hb = h + b * hstride;
+  c128* cb = cpend + b * cstride;
+  BasisCtl& C = ctl[b];
+  double yy = 0.0, gg = 0.0;
+  for (int i = t; i < nblk; i += kThreads) yy += ypart[static_cast<long long>(b) * nblk + i];
+  if (t < j) {
+    const c128 v = hb[t];
+    gg = v.re * v.re + v.im * v.im;
+    gs[t] = v;
+  }
+  const double at = atilde[b];
+  const c128 gam = hb[j];
+  const double cprev = j > 0 ? cb[j - 1].re : 0.0;
+  const double bprev = C.beta_prev;
+  const int target = C.target;
+  yy = block_sum(yy, sm);   // ends with a barrier: every read above precedes the writes below
+  gg = block_sum(gg, sm);
+  const c128 glast = cmake(gam.re - at, gam.im);
+  gg += glast.re * glast.re + glast.im * glast.im;
+  const bool lead = blockIdx.x == 0;
+  if (lead && t == 0) alpha[b * astride + j] = gam.re - bprev * cprev;
+  if (j + 1 >= target) return;                 // the trajectory's last vector: no beta_j
+  const double b2 = yy - gg;
+  bool fix = !(b2 > kCancel * yy);
+  if (!fix) fix = gg > kFixThreshold * kFixThreshold * b2;
+  if (!fix) {
+    if (!lead) return;
+    const double bj = std::sqrt(b2);
+    if (t == 0) beta[b * astride + j] = bj;
     if (bj <= tol) {
-      ctl[b].active = 0;
-      ctl[b].size = j + 1;
+      if (t == 0) {
+        C.active = 0;
+        C.size = j + 1;
+      }
+      return;
+    }
+    const double inv = 1.0 / bj;
+    if (t < j) cb[t] = cscale(gs[t], inv);
+    if (t == 0) {
+      cb[j] = cscale(glast, inv);
+      C.scale = inv;
+      C.beta_prev = bj;
+    }
+    return;
+  }
+  // explicit correction: y -= V_<=j g
+  if (t == 0) gs[j] = glast;
+  __syncthreads();
+  const int k = j + 1;
+  const c128* Vb = V + b * vstride;
+  c128* y = U + b * n;
+  double s = 0.0;
+  for (long long i = static_cast<long long>(blockIdx.x) * kThreads + t; i < n;
+       i += static_cast<long long>(gridDim.x) * kThreads) {
+    c128 yv = y[i];
+    for (int c = 0; c < k; ++c) yv = cfms(Vb[static_cast<long long>(c) * n + i], gs[c], yv);
+    y[i] = yv;
+    s = fma(yv.re, yv.re, fma(yv.im, yv.im, s));
+  }
+  s = block_sum(s, sm);
+  if (t == 0) {
+    fpart[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
+    __threadfence();
+    is_last = atomicAdd(&counter[b], 1u) == gridDim.x - 1;
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const volatile double* fp = fpart + static_cast<long long>(b) * gridDim.x;
+  double tot = 0.0;
+  for (int i = t; i < static_cast<int>(gridDim.x); i += kThreads) tot += fp[i];
+  tot = block_sum(tot, sm);
+  if (t < k) cb[t] = cmake(0.0, 0.0);
+  if (t == 0) {
+    counter[b] = 0;
+    const double bj = std::sqrt(tot);
+    beta[b * astride + j] = bj;
+    if (bj <= tol) {
+      C.active = 0;
+      C.size = j + 1;
     } else {
-      ctl[b].scale = 1.0 / bj;
-      ctl[b].beta_prev = bj;
+      C.scale = 1.0 / bj;
+      C.beta_prev = bj;
     }
   }
 }
@@ -459,14 +542,18 @@ extern "C" int rpm_singlet_population(const void* d_x, long long nn, int batch, 
 
 namespace {
 struct BasisWs {
-  c128* dots;        // [batch][units] matvec partials of alpha_j
+  c128* dots;        // [batch][units] matvec partials of alpha~_j
   int units;
   c128* w[2];        // [batch][n] each
-  c128* partial;     // [batch][m][nblk]
+  c128* partial;     // [batch][m][npass] k_lcgs_pass column partials (column j: gamma)
   long long pstride;
-  int nblk;
-  c128* h;           // [batch][m]
-  double* npart;     // [batch][nupd]
+  int npass;
+  double* ypart;     // [batch][npass] |y|^2 partials
+  c128* h;           // [batch][m] reduced column sums
+  c128* cpend;       // [batch][m] pending correction c_{j+1}
+  double* fpart;     // [batch][kFixBlocks]
+  double* atilde;    // [batch] alpha~_j
+  unsigned* counter; // [batch]
   double* sq;        // [batch][nchunk]
   BasisCtl* ctl;     // [batch]
   size_t bytes;
@@ -474,8 +561,7 @@ struct BasisWs {
 
 BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   const int nchunk = ceil_div(n, kChunk);
-  const int gsb = ceil_div(n, static_cast<long long>(kDotPer) * kWave);   // k_gs_dots blocks
-  const int nupd = ceil_div(n, kUpdChunk);
+  const int npass = ceil_div(n, static_cast<long long>(kPassElems));
   size_t off = 0;
   auto take = [&](size_t b) {
     size_t o = off;
@@ -484,23 +570,36 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   };
   const size_t B = static_cast<size_t>(batch);
   const size_t o_w0 = take(B * n * 16), o_w1 = take(B * n * 16);
-  const size_t o_p = take(B * m * gsb * 16), o_h = take(B * m * 16), o_np = take(B * nupd * 8);
+  const size_t o_p = take(B * m * npass * 16), o_yp = take(B * npass * 8), o_h = take(B * m * 16);
+  const size_t o_c = take(B * m * 16), o_fp = take(B * kFixBlocks * 8), o_cnt = take(B * sizeof(unsigned));
+  const size_t o_at = take(B * 8);
   const size_t o_sq = take(B * nchunk * 8), o_ctl = take(B * sizeof(BasisCtl));
   const size_t o_dots = take(B * static_cast<size_t>(units) * 16);
   BasisWs w;
   w.w[0] = reinterpret_cast<c128*>(base + o_w0);
   w.w[1] = reinterpret_cast<c128*>(base + o_w1);
   w.partial = reinterpret_cast<c128*>(base + o_p);
-  w.pstride = static_cast<long long>(m) * gsb;
-  w.nblk = gsb;
+  w.pstride = static_cast<long long>(m) * npass;
+  w.npass = npass;
+  w.ypart = reinterpret_cast<double*>(base + o_yp);
   w.h = reinterpret_cast<c128*>(base + o_h);
-  w.npart = reinterpret_cast<double*>(base + o_np);
+  w.cpend = reinterpret_cast<c128*>(base + o_c);
+  w.fpart = reinterpret_cast<double*>(base + o_fp);
+  w.atilde = reinterpret_cast<double*>(base + o_at);
+  w.counter = reinterpret_cast<unsigned*>(base + o_cnt);
   w.sq = reinterpret_cast<double*>(base + o_sq);
   w.ctl = reinterpret_cast<BasisCtl*>(base + o_ctl);
   w.dots = reinterpret_cast<c128*>(base + o_dots);
   w.units = units;
   w.bytes = off;
   return w;
+}
+
+template <int CPW>
+void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int j, int m, bool fuse_alpha, c128* out,
+                 int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_lcgs_pass<CPW>, dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j, w.cpend, m,
+                     w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, w.ypart, w.ctl);
 }
 
 // Shared implementation of the single and batched basis builders.
@@ -513,20 +612,17 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch, apply_units(op));
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "Lanczos workspace %zu < %zu", ws_bytes, w.bytes);
   const int nchunk = ceil_div(n, kChunk);
-  const int nupd = ceil_div(n, kUpdChunk);
   const int count = static_cast<int>(max_target < n ? max_target : n);
-  const int dot_per = kDotPer;
-  const int dot_blocks = ceil_div(n, static_cast<long long>(dot_per) * kWave);
-  if (dot_blocks > w.nblk) return fail(RPM_E_INVALID, "internal: dots partial buffer too small");
   hipLaunchKernelGGL(k_sumsq, dim3(nchunk, batch), dim3(kThreads), 0, s, states, n, sld, w.sq, d_targets);
   RPM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, d_targets, count, n);
+  hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, w.counter, d_targets,
+                     count, n);
   RPM_LAUNCH_CHECK();
   for (int j = 0; j < count; ++j) {
     const c128* in = j == 0 ? states : w.w[j % 2];
     c128* out = w.w[(j + 1) % 2];
-    // matvec with the normalisation, the copy V[j] = s in, the three-term subtraction
-    // - beta_{j-1} V[j-1] and the per-unit partials of alpha_j = <v_j, H v_j>
+    // matvec x_j = s in: the copy V[j] = x_j, out = H x_j - beta_{j-1} V[j-1] and the per-unit
+    // partials of alpha~_j = <x_j, H x_j>
     ApplyCtl A;
     A.ctl = w.ctl;
     A.j = j;
@@ -539,36 +635,29 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     A.dots_ld = w.units;
     int rc = operator_apply_ex(op, in, out, batch, j == 0 ? sld : n, A, s);
     if (rc) return rc;
-    // A single trajectory forms alpha_j inside k_gs_dots (one launch fewer per step). For a batch
+    // A single trajectory forms alpha~_j inside the pass (one launch fewer per step). For a batch
     // the per-block recomputation costs more than the launch, so k_alpha runs; both sum in the
     // same order, so the bases stay bitwise equal.
-    const bool fuse_alpha = batch == 1 && j < count - 1;
+    const bool fuse_alpha = batch == 1;
     if (!fuse_alpha) {
-      hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, alpha, m, j, w.ctl);
+      hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, w.atilde, j, w.ctl);
       RPM_LAUNCH_CHECK();
     }
-    if (j == count - 1) break;
-    const int k = j + 1;
-    // pass 1: (alpha_j,) w -= alpha_j V[j], h = V^H w
-    // one vector: 2 columns per wave over ceil(k / 8) column groups, so that it fills the chip;
-    // a batch: all columns in one block (same partials either way)
-    if (batch == 1)
-      hipLaunchKernelGGL(k_gs_dots<2>, dim3(dot_blocks, 1, ceil_div(k, kDotWaves * 2)), dim3(kDotWaves * kWave), 0, s,
-                         V, vstride, n, k, dot_per, alpha, m, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
-                         w.pstride, j, 1, w.ctl);
+    // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
+    if (j <= kPassWaves * 4)
+      launch_pass<4>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
+    else if (j <= kPassWaves * 8)
+      launch_pass<8>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
     else
-      hipLaunchKernelGGL(k_gs_dots<kBatchCpw>, dim3(dot_blocks, batch, ceil_div(k, kDotWaves * kBatchCpw)),
-                         dim3(kDotWaves * kWave), 0, s, V, vstride, n, k, dot_per, alpha, m,
-                         fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, j, 1, w.ctl);
+      launch_pass<16>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(k, batch), dim3(kThreads), 0, s, w.partial, dot_blocks, w.pstride, w.h, m,
-                       nullptr, m, 0, j, 1, w.ctl);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
+                       nullptr, m, 0, j, 0, w.ctl);
     RPM_LAUNCH_CHECK();
-    // pass 2: w -= V h, ||w||
-    hipLaunchKernelGGL(k_update, dim3(nupd, batch), dim3(kThreads), 0, s, V, vstride, n, k, w.h, m, alpha, m, out,
-                       w.npart, j, w.ctl);
-    RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_finish_beta, dim3(batch), dim3(kThreads), 0, s, w.npart, nupd, beta, m, j, tol, w.ctl);
+    // alpha_j, beta_j, c_{j+1}; the explicit correction when it is due
+    hipLaunchKernelGGL(k_lcgs_finish, dim3(j == count - 1 ? 1 : kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n,
+                       w.ypart, w.npass, w.h, m, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter,
+                       w.ctl);
     RPM_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_basis_info, dim3(ceil_div(batch, 64)), dim3(64), 0, s, w.ctl, info, batch);
