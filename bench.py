@@ -157,7 +157,8 @@ def propagate_probe(h, args, horizon=None) -> dict:
     cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
     times = np.arange(int(round(horizon / 0.00125)) + 1) * 0.00125
     prop = core.make_propagator(h, cfg, probe_horizon=float(times[-1]))
-    s = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 0)
+    # the initial state is resident in HBM before the timed region
+    s = torch.from_numpy(core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 0)).cuda()
     obs = core.SingletPopulationObservable(h.dimensions)
     prop.propagate(s, times[:3], obs)          # warm-up (allocations, first launches)
     torch.cuda.synchronize()
@@ -196,10 +197,11 @@ def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
         th, ph, f = orients[i % len(orients)]
         hs.append(core.build_hamiltonian(th, f, phi_deg=ph))
         states.append(core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i // len(orients)))
-    states = np.stack(states)
+    # the initial states are resident in HBM before the timed region
+    states = torch.from_numpy(np.stack(states)).cuda()
     pairs = batch.pair_matrices(h, hs)
     bp = batch.BatchedLanczosPropagator.like(core.make_propagator(h, cfg, probe_horizon=float(times[-1])))
-    bp.propagate(states[:2], times[:3], pairs=pairs[:2])      # warm-up
+    bp.propagate(states, times[:3], pairs=pairs)      # warm-up at the full batch (allocations, first launches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     res = bp.propagate(states, times, pairs=pairs)
