@@ -158,21 +158,47 @@ __global__ void __launch_bounds__(256) k_singlet_diff(const c128* __restrict__ u
   chi[b * cld + x] = csub(p, q);
 }
 
-// Costate seed of one Chebyshev order m:
-//   out = cg g + P sum_j G[j] u_j,  (P v)[1] = -(P v)[2] = (v[1] - v[2]) / 2,
-// with the u_j given through chi_j = u_j[1] - u_j[2] (chi: nj blocks of cstride).
-__global__ void __launch_bounds__(256) k_seed(const c128* __restrict__ g, c128 cg, const c128* __restrict__ chi,
-                                              long long cstride, long long cld, int nj, const c128* __restrict__ G,
-                                              double mult, c128* __restrict__ out, long long ld, long long nn) {
+// Singlet sums of the costate seeds of every Chebyshev order of a step, in one pass over the
+// chi_j = u_j[1] - u_j[2] (chi: nj blocks of cstride, trajectory b at b * cld):
+//   S_m = sum_j G[m][j] chi_j,  m in [blockIdx.z * kSeedRows, ...) and m < nm,
+// summed over j in ascending order for every m.
+constexpr int kSeedRows = 8;
+__global__ void __launch_bounds__(256) k_seed_sums(const c128* __restrict__ chi, long long cstride, long long cld,
+                                                   int nj, const c128* __restrict__ G, int gstride, int nm,
+                                                   c128* __restrict__ S, long long nn) {
   const long long b = blockIdx.y;
   const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-  __shared__ c128 gs[128];
-  for (int j = threadIdx.x; j < nj; j += 256) gs[j] = G[j];
+  const int m0 = static_cast<int>(blockIdx.z) * kSeedRows;
+  __shared__ c128 gs[kSeedRows][128];
+  for (int t = threadIdx.x; t < kSeedRows * nj; t += 256) {
+    const int i = t / nj, j = t % nj;
+    gs[i][j] = m0 + i < nm ? G[static_cast<long long>(m0 + i) * gstride + j] : cmake(0.0, 0.0);
+  }
   __syncthreads();
   if (x >= nn) return;
-  c128 s{0.0, 0.0};
-  for (int j = 0; j < nj; ++j) s = cfma(gs[j], chi[j * cstride + b * cld + x], s);
-  s = cscale(s, 0.5 * mult);
+  c128 acc[kSeedRows];
+#pragma unroll
+  for (int i = 0; i < kSeedRows; ++i) acc[i] = cmake(0.0, 0.0);
+  for (int j = 0; j < nj; ++j) {
+    const c128 c = chi[j * cstride + b * cld + x];
+#pragma unroll
+    for (int i = 0; i < kSeedRows; ++i) acc[i] = cfma(gs[i][j], c, acc[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < kSeedRows; ++i)
+    if (m0 + i < nm) S[(m0 + i) * cstride + b * cld + x] = acc[i];
+}
+
+// Costate seed of one Chebyshev order m:
+//   out = cg g + P sum_j G[m][j] u_j,  (P v)[1] = -(P v)[2] = (v[1] - v[2]) / 2,
+// with the singlet sum S_m = sum_j G[m][j] chi_j from k_seed_sums.
+__global__ void __launch_bounds__(256) k_seed(const c128* __restrict__ g, c128 cg, const c128* __restrict__ Sm,
+                                              long long cld, double mult, c128* __restrict__ out, long long ld,
+                                              long long nn) {
+  const long long b = blockIdx.y;
+  const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+  if (x >= nn) return;
+  const c128 s = cscale(Sm[b * cld + x], 0.5 * mult);
   const c128* gb = g + b * ld;
   c128* ob = out + b * ld;
   ob[x] = cfma(cg, gb[x], cmake(0.0, 0.0));
@@ -244,9 +270,9 @@ extern "C" size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch,
   if (!op || batch < 1 || kmax < 2) return 0;
   const size_t vec = ((static_cast<size_t>(op->n) * batch * sizeof(c128)) + 255) & ~size_t(255);
   const size_t qvec = ((static_cast<size_t>(op->n / 4) * batch * sizeof(c128)) + 255) & ~size_t(255);
-  // psi', g (2), ring a/b, seed z, u_1..u_K; then chi_0..chi_K
+  // psi', g (2), ring a/b, seed z, u_1..u_K; then chi_0..chi_K and the seed sums S_0..S_K
   const size_t vecs = 6 + static_cast<size_t>(kmax);
-  return vecs * vec + (kmax + 1) * qvec + rpm_cross_density_workspace_bytes(op->n / 4, batch) + 256;
+  return vecs * vec + 2 * (kmax + 1) * qvec + rpm_cross_density_workspace_bytes(op->n / 4, batch) + 256;
 }
 
 extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro, double shift,
@@ -287,9 +313,10 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
   std::vector<c128*> u(kmax + 1);
   for (int i = 1; i <= kmax; ++i) u[i] = vecp(5 + i);
   c128* chi = reinterpret_cast<c128*>(base + vec * (6 + kmax));
+  c128* ssum = reinterpret_cast<c128*>(base + vec * (6 + kmax) + qvec * (kmax + 1));
   const long long cstride = static_cast<long long>(qvec / sizeof(c128));   // per order j
-  C.dens_ws = reinterpret_cast<c128*>(base + vec * (6 + kmax) + qvec * (kmax + 1));
-  C.dens_bytes = ws_bytes - (vec * (6 + kmax) + qvec * (kmax + 1));
+  C.dens_ws = reinterpret_cast<c128*>(base + vec * (6 + kmax) + 2 * qvec * (kmax + 1));
+  C.dens_bytes = ws_bytes - (vec * (6 + kmax) + 2 * qvec * (kmax + 1));
   if (C.dens_bytes < rpm_cross_density_workspace_bytes(C.nn, batch))
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: workspace too small");
   {
@@ -324,10 +351,14 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
                          C.nn);
       RPM_LAUNCH_CHECK();
     }
-    // Clenshaw over the costate with vector seeds z_m = conj(c_m) g + mult P sum_j G[m][j] u_j
+    // Clenshaw over the costate with vector seeds z_m = conj(c_m) g + mult P sum_j G[m][j] u_j; the
+    // singlet sums of all orders in one pass over the chi_j
+    hipLaunchKernelGGL(k_seed_sums, dim3(nqb, batch, ceil_div(K + 1, kSeedRows)), dim3(256), 0, C.s, chi, cstride,
+                       C.nn, K + 1, G, kmax + 1, K + 1, ssum, C.nn);
+    RPM_LAUNCH_CHECK();
     auto seed = [&](int m) {
-      hipLaunchKernelGGL(k_seed, dim3(nqb, batch), dim3(256), 0, C.s, g_a, conjc(cf + 2 * m), chi, cstride, C.nn,
-                         K + 1, G + static_cast<size_t>(m) * (kmax + 1), mult[k], z, C.n, C.nn);
+      hipLaunchKernelGGL(k_seed, dim3(nqb, batch), dim3(256), 0, C.s, g_a, conjc(cf + 2 * m), ssum + m * cstride,
+                         C.nn, mult[k], z, C.n, C.nn);
       return hipGetLastError() == hipSuccess ? RPM_OK : fail(RPM_E_HIP, "k_seed launch failed");
     };
     if ((rc = seed(K))) return rc;
