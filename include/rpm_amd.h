@@ -127,7 +127,9 @@ int rpm_operator_kernel(const rpm_operator* op);
 size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m);
 
 /*
- * CGS2 Lanczos basis of d_state (not modified), up to `target` vectors.
+ * Lanczos basis of d_state (not modified), up to `target` vectors, fully
+ * reorthogonalised (three-term step plus one full projection per vector, the
+ * projection applied one step late so that each step reads the basis once).
  * d_basis: target * N complex (vector j at offset j*N).
  * d_alpha[target], d_beta[target] (FP64, device), d_info[4] (int, device):
  *   info[0] = basis size actually built (< target on breakdown),
@@ -140,7 +142,8 @@ int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, v
                       void* stream);
 
 /*
- * Batched CGS2 basis: trajectory b < batch starts from d_states + b*ld (ld == N) and
+ * Batched basis (as rpm_lanczos_basis): trajectory b < batch starts from
+ * d_states + b*ld (ld == N) and
  * builds min(d_targets[b], max_target) vectors into d_basis + b*m*N (vector j
  * at + j*N).  d_targets[b] <= 0 leaves trajectory b untouched (its basis,
  * tridiagonal, norm and info keep their values).  d_pairs (nullable): per

@@ -7,7 +7,8 @@ coroutine per trajectory, and every device operation they ask for (Lanczos
 basis, candidate errors, projected outputs, endpoint update) is served for
 all trajectories that want it in one batched launch:
 
-* ``rpm_lanczos_basis_batch`` -- CGS2 bases of all trajectories that need one,
+* ``rpm_lanczos_basis_batch`` -- fully reorthogonalised Lanczos bases of all
+  trajectories that need one,
   each with its own target size, breakdown flag and electron 4x4 (so one
   launch mixes field orientations and strengths);
 * ``rpm_krylov_tasks`` -- every candidate and output-time evaluation of the

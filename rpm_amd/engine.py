@@ -6,7 +6,8 @@ in librpm_amd.so (hand-written HIP for gfx950):
 
 * ``MatrixFreeHamiltonian`` / ``CompiledFullPairHamiltonian`` -> fused H·psi
   kernels (pair stencil or generic local-term gather);
-* ``AdaptiveLanczosPropagator`` -> device CGS2 Lanczos basis, device
+* ``AdaptiveLanczosPropagator`` -> device Lanczos basis (fully
+  reorthogonalised, one read of the basis per step), device
   tridiagonal eigensolver / projected exponential, device singlet projection
   and endpoint update.  The host keeps only the step-control decisions of
   R/rpm_statevector_engine.py:717-933, which read a few scalars per step.
@@ -624,7 +625,7 @@ class AdaptiveLanczosPropagator:
         return self._ws
 
     def _basis(self, state, target):
-        """Device CGS2 basis; returns (size, alpha, beta) on the host and fills the workspace."""
+        """Device Lanczos basis; returns (size, alpha, beta) on the host and fills the workspace."""
         ws = self._workspace()
         op = self.hamiltonian._device_op()
         count = min(int(target), state.numel())
