@@ -133,6 +133,7 @@ size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m);
  * d_basis: target * N complex (vector j at offset j*N).
  * d_alpha[target], d_beta[target] (FP64, device), d_info[4] (int, device):
  *   info[0] = basis size actually built (< target on breakdown),
+ *   info[2] = explicit corrections of the lagged projection applied (diagnostic),
  * d_norm (device double): receives ||state||.
  * Matvec count on the host side equals info[0] (one per basis vector).
  */
@@ -149,7 +150,8 @@ int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, v
  * tridiagonal, norm and info keep their values).  d_pairs (nullable): per
  * trajectory electron 4x4 as in rpm_operator_apply_pairs.  d_alpha, d_beta:
  * batch x m (row b at b*m); d_info: batch x 4 (info[4b] = size built,
- * info[4b+1] = 1 if no breakdown); d_norm[batch].  max_target <= m <= 64.
+ * info[4b+1] = 1 if no breakdown, info[4b+2] = explicit corrections); d_norm[batch].
+ * max_target <= m <= 64.
  */
 size_t rpm_lanczos_batch_workspace_bytes(const rpm_operator* op, int m, int batch);
 int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_states, long long ld,

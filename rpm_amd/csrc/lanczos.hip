@@ -113,7 +113,7 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
     int count = targets ? min(targets[b], uniform) : uniform;
     if (count > n) count = static_cast<int>(n);
     counter[b] = 0;
-    ctl[b].fix = 0;
+    ctl[b].nfix = 0;
     if (count <= 0) {
       // not part of this call: its norm, basis and tridiagonal stay as they are
       ctl[b].active = 0;
@@ -379,6 +379,7 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   if (t < k) cb[t] = cmake(0.0, 0.0);
   if (t == 0) {
     counter[b] = 0;
+    C.nfix += 1;
     const double bj = std::sqrt(tot);
     beta[b * astride + j] = bj;
     if (bj <= tol) {
@@ -396,6 +397,7 @@ __global__ void k_basis_info(const BasisCtl* ctl, int* info, int batch) {
   if (b < batch && ctl[b].target >= 0) {
     info[b * 4 + 0] = ctl[b].size;
     info[b * 4 + 1] = ctl[b].active;
+    info[b * 4 + 2] = ctl[b].nfix;
   }
 }
 

@@ -118,7 +118,7 @@ struct BasisCtl {
   int active;       // 0 once the basis broke down (or the vector is idle)
   int size;         // basis size built so far / final
   int target;       // requested basis size
-  int fix;          // 1 while the pending correction of the next vector awaits k_lcgs_fix
+  int nfix;         // explicit corrections of the lagged projection applied in this basis
   double scale;     // 1/norm of the vector the next matvec normalises
   double beta_prev; // beta_{j-1} of the Lanczos three-term step (0 for j = 0)
 };

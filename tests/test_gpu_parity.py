@@ -158,6 +158,31 @@ def test_lanczos_basis_matches_oracle_cgs2():
     assert np.abs(V.conj().T @ V - np.eye(size)).max() < 1e-13
 
 
+def test_long_production_basis_with_explicit_corrections(production_pair):
+    """64 Lanczos vectors of the production operator (dim 221,184) against the oracle's CGS2.
+
+    The device applies the full projection one step late; the pending correction grows by about
+    |H| / beta (x2) per step, so a 64-vector basis passes the explicit-correction threshold (1e-6)
+    at least once: info[2] counts those corrections, and the basis must still match.
+    """
+    print("case: production basis m=64", flush=True)
+    h, o = production_pair
+    state = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 3)
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=32, maximum_dimension=64, embedded_offset=4)
+    oprop = oengine.AdaptiveLanczosPropagator(o, minimum_dimension=32, maximum_dimension=64, embedded_offset=4)
+    size = prop._basis(torch.from_numpy(state).cuda(), 64)
+    ws = prop._workspace()
+    assert size == 64
+    assert int(ws.info[2].item()) >= 1
+    ov, oa, ob = oprop._basis(state, 64)
+    np.testing.assert_allclose(ws.alpha[:size].cpu().numpy(), oa, rtol=0, atol=1e-10 * np.abs(oa).max())
+    np.testing.assert_allclose(ws.beta[: size - 1].cpu().numpy(), ob, rtol=1e-11)
+    V = ws.V[:size].cpu().numpy().T
+    assert np.abs(V - ov).max() < 1e-10
+    gram = V.conj().T @ V
+    assert np.abs(gram - np.eye(size)).max() < 1e-13
+
+
 def test_breakdown_on_invariant_subspace():
     # H with a 3-dimensional invariant subspace containing the start vector -> basis of size 3
     dims = (2, 2, 2)
