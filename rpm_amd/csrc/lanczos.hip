@@ -175,8 +175,9 @@ __global__ void __launch_bounds__(kThreads) k_alpha(const c128* __restrict__ dot
 //
 // Step j works on the materialised basis v_0 .. v_{j-1} and the vector x_j that the matvec
 // consumed, which is v_j plus a pending O(eps) combination of the basis, v_j = x_j - V_<j c_j
-// (c_j from step j-1). With u = H x_j - beta_{j-1} v_{j-1} (the matvec), one pass over V_<j
-//   * materialises v_j = x_j - V_<j c_j in V[j],
+// (c_j from step j-1). The matvec writes H x_j; one pass over V_<j
+//   * forms u = H x_j - beta_{j-1} v_{j-1} as it stages H x_j,
+//   * materialises v_j = x_j - V_<j c_j in V[j] (x_j = s in, from the matvec's input),
 //   * forms D = V_<j^H u, gamma = v_j^H u and y = u - alpha~ v_j (alpha~ = x_j^H H x_j from the
 //     matvec's partials), with |y|^2.
 // Then (k_lcgs_finish), with H V_<j = V_<=j T + O(eps):
@@ -204,6 +205,7 @@ static_assert(kPassWaves * kWave == kThreads, "k_lcgs_pass shares the kThreads-w
 // same for every CPW and batch size.
 template <int CPW>
 __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, long long vstride, long long n, int j,
+                                                        const c128* __restrict__ in, long long ild,
                                                         const c128* __restrict__ cpend, int cstride,
                                                         double* __restrict__ atilde,
                                                         const c128* __restrict__ adots, int units, c128* __restrict__ U,
@@ -229,7 +231,18 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
   c128* u = U + b * n;
   const long long x0 = static_cast<long long>(blockIdx.x) * kPassElems;
   const int cnt = static_cast<int>(n - x0 < kPassElems ? n - x0 : kPassElems);
-  for (int e = threadIdx.x; e < cnt; e += kThreads) s_u[e] = u[x0 + e];
+  // u = H x_j - beta_{j-1} v_{j-1}, the three-term step's beta term
+  const double bprev = ctl[b].beta_prev;
+  const c128* vprev = j > 0 ? Vb + static_cast<long long>(j - 1) * n : nullptr;
+  for (int e = threadIdx.x; e < cnt; e += kThreads) {
+    c128 uu = u[x0 + e];
+    if (vprev) {
+      const c128 p = vprev[x0 + e];
+      uu.re = fma(-bprev, p.re, uu.re);
+      uu.im = fma(-bprev, p.im, uu.im);
+    }
+    s_u[e] = uu;
+  }
   if (threadIdx.x < j) s_c[threadIdx.x] = cpend[b * cstride + threadIdx.x];
   __syncthreads();
   const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
@@ -264,11 +277,13 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
   }
   __syncthreads();
   c128* vj = Vb + static_cast<long long>(j) * n;
+  const c128* xin = in + b * ild;
+  const double sc = ctl[b].scale;
   c128 g = cmake(0.0, 0.0);
   double yy = 0.0;
   for (int e = threadIdx.x; e < cnt; e += kThreads) {
     const c128 corr = cadd(cadd(cadd(s_corr[0][e], s_corr[1][e]), s_corr[2][e]), s_corr[3][e]);
-    const c128 v = csub(vj[x0 + e], corr);
+    const c128 v = csub(cscale(xin[x0 + e], sc), corr);
     vj[x0 + e] = v;
     const c128 uu = s_u[e];
     const c128 y = cmake(fma(-at, v.re, uu.re), fma(-at, v.im, uu.im));
@@ -598,10 +613,10 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
 }
 
 template <int CPW>
-void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int j, int m, bool fuse_alpha, c128* out,
-                 int batch, hipStream_t s) {
-  hipLaunchKernelGGL(k_lcgs_pass<CPW>, dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j, w.cpend, m,
-                     w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, w.ypart, w.ctl);
+void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int j, const c128* in, long long ild, int m,
+                 bool fuse_alpha, c128* out, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_lcgs_pass<CPW>, dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j, in, ild, w.cpend,
+                     m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride, w.ypart, w.ctl);
 }
 
 // Shared implementation of the single and batched basis builders.
@@ -623,19 +638,16 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   for (int j = 0; j < count; ++j) {
     const c128* in = j == 0 ? states : w.w[j % 2];
     c128* out = w.w[(j + 1) % 2];
-    // matvec x_j = s in: the copy V[j] = x_j, out = H x_j - beta_{j-1} V[j-1] and the per-unit
-    // partials of alpha~_j = <x_j, H x_j>
+    // matvec on x_j = s in: out = H x_j and the per-unit partials of alpha~_j = <x_j, H x_j>; the
+    // pass subtracts beta_{j-1} V[j-1] and writes V[j]
+    const long long ild = j == 0 ? sld : n;
     ApplyCtl A;
     A.ctl = w.ctl;
     A.j = j;
-    A.copy = V + static_cast<long long>(j) * n;
-    A.copy_ld = vstride;
     A.pairs = pairs;
-    A.prev = j > 0 ? V + static_cast<long long>(j - 1) * n : nullptr;
-    A.prev_ld = vstride;
     A.dots = w.dots;
     A.dots_ld = w.units;
-    int rc = operator_apply_ex(op, in, out, batch, j == 0 ? sld : n, A, s);
+    int rc = operator_apply_ex(op, in, out, batch, ild, A, s);
     if (rc) return rc;
     // A single trajectory forms alpha~_j inside the pass (one launch fewer per step). For a batch
     // the per-block recomputation costs more than the launch, so k_alpha runs; both sum in the
@@ -647,11 +659,11 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     }
     // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
     if (j <= kPassWaves * 4)
-      launch_pass<4>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
+      launch_pass<4>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else if (j <= kPassWaves * 8)
-      launch_pass<8>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
+      launch_pass<8>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else
-      launch_pass<16>(w, V, vstride, n, j, m, fuse_alpha, out, batch, s);
+      launch_pass<16>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
                        nullptr, m, 0, j, 0, w.ctl);
