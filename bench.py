@@ -149,13 +149,13 @@ def lanczos_algorithmic_bytes(schedule, n: int) -> int:
 
 def lanczos_device_bytes(schedule, n: int) -> int:
     """HBM bytes this implementation moves for the same schedule (its lagged one-pass orthogonalisation):
-    per Lanczos step j of a basis, 16 N (4 [matvec: in, out, the copy V[j], v_{j-1}] + j + 4 [one pass
-    over V[0..j), x_j and u read, v_j and y written]); per accepted step the same endpoint and singlet
-    terms as lanczos_algorithmic_bytes. The rare explicit corrections are not counted."""
+    per Lanczos step j of a basis, 16 N (2 [matvec: in, H x_j] + j + 4 [one pass over V[0..j), x_j and
+    H x_j read, v_j and y written]); per accepted step the same endpoint and singlet terms as
+    lanczos_algorithmic_bytes. The rare explicit corrections are not counted."""
     total = 0
     for _, accepted, selected, sizes in schedule:
         for m in sizes:
-            total += 16 * n * sum(8 + j for j in range(m))
+            total += 16 * n * sum(6 + j for j in range(m))
         if accepted:
             total += 16 * n * (selected + 1) + 8 * n * selected
     return total
