@@ -5,6 +5,7 @@ Bars: H·psi relative L2 error <= 1e-13; Lanczos coefficients / basis to
 same seeds with identical step schedules.
 """
 
+import ctypes
 import math
 
 import numpy as np
@@ -18,6 +19,7 @@ from oracle import core as ocore  # noqa: E402
 from oracle import dense as odense  # noqa: E402
 from oracle import engine as oengine  # noqa: E402
 from oracle import model as omodel  # noqa: E402
+from rpm_amd import _native as nat  # noqa: E402
 from rpm_amd import core, engine, model, workloads  # noqa: E402
 
 HPSI_TOL = 1e-13
@@ -181,6 +183,48 @@ def test_long_production_basis_with_explicit_corrections(production_pair):
     assert np.abs(V - ov).max() < 1e-10
     gram = V.conj().T @ V
     assert np.abs(gram - np.eye(size)).max() < 1e-13
+
+
+def test_basis_captured_in_hip_graph_replays_bitwise(production_pair):
+    """The basis builder launches only kernels (no host sync, no memset), so a caller can capture it in
+    a HIP graph: every replay, with eager work between replays, equals the eager basis bitwise."""
+    print("case: basis in a HIP graph, m=24", flush=True)
+    h, _ = production_pair
+    prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=16, maximum_dimension=24, embedded_offset=4)
+    x = torch.from_numpy(core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 5)).cuda()
+    y = torch.from_numpy(core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, 6)).cuda()
+    m = 24
+    prop._basis(x, m)
+    ws = prop._workspace()
+    op = h._device_op()
+    lib = nat.load()
+
+    def build(state):
+        nat.check(lib.rpm_lanczos_basis(
+            ctypes.c_void_p(op.handle), ctypes.c_void_p(nat.ptr(state)), m, ctypes.c_void_p(nat.ptr(ws.V)),
+            ctypes.c_void_p(nat.ptr(ws.alpha)), ctypes.c_void_p(nat.ptr(ws.beta)), ctypes.c_void_p(nat.ptr(ws.info)),
+            ctypes.c_void_p(nat.ptr(ws.norm)), 1e-14, ctypes.c_void_p(nat.ptr(ws.ws)), ws.ws.numel(),
+            ctypes.c_void_p(nat.stream_ptr())))
+
+    build(x)
+    torch.cuda.synchronize()
+    ref = (ws.alpha[:m].clone(), ws.beta[:m - 1].clone(), ws.V[:m].clone())
+    side = torch.cuda.Stream()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.stream(side):
+        build(x)
+        torch.cuda.synchronize()
+        with torch.cuda.graph(graph, stream=side):
+            build(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        build(y)                      # eager work on the same workspace between replays
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(ws.alpha[:m], ref[0])
+        assert torch.equal(ws.beta[:m - 1], ref[1])
+        assert torch.equal(ws.V[:m], ref[2])
 
 
 def test_breakdown_on_invariant_subspace():
