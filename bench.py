@@ -385,7 +385,8 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched-yield and gradient probes")
-    ap.add_argument("--kernel", type=int, default=0, help="0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled")
+    ap.add_argument("--kernel", type=int, default=0,
+                    help="0 tuned at run time (rpm_amd.autotune), 1 gather, 2 register-tiled, 3 LDS-tiled")
     ap.add_argument("--operator", default="production", choices=("production", "random"),
                     help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
@@ -426,9 +427,14 @@ def main() -> int:
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()
-    h._device_op().set_kernel(args.kernel)
     n = h.dimension
     B = args.batch
+    tuned = None
+    if args.kernel == 0 and args.mode == "hpsi":
+        from rpm_amd import autotune
+        tuned = autotune.hpsi_kernel(h, B)      # times the H·ψ kernels at this batch and keeps the fastest
+    else:
+        h._device_op().set_kernel(args.kernel)
     print(f"[bench] building {B} synthetic vectors of dim {n}", file=sys.stderr, flush=True)
     x = torch.from_numpy(workloads.normal_vectors(n, B, seed=0)).cuda()
     y = torch.empty_like(x)
@@ -484,6 +490,10 @@ def main() -> int:
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(B) if args.operator == "production" else None},
     }
+    if tuned is not None:
+        from rpm_amd.autotune import HPSI_KERNELS
+        result["autotune"] = {"hpsi_kernel": tuned["name"],
+                              "ms_per_launch": {HPSI_KERNELS[k]: ms for k, ms in tuned["ms_per_launch"].items()}}
     if not args.no_extras:
         # the same kernel at other batch sizes: B <= 32 fits the 256 MB Infinity Cache, B >= 128 is HBM-bound
         del y

@@ -443,7 +443,7 @@ class BatchedLanczosPropagator:
 
 
 def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, phis_deg, fields_uT, times_us,
-                       lifetime_us, *, batch_size: int = 32, progress_callback=None) -> np.ndarray:
+                       lifetime_us, *, batch_size=32, progress_callback=None) -> np.ndarray:
     """Singlet yields of every (random-phase state, field direction, field strength): (S, D, F).
 
     The directional-map workload: many random-phase vectors x field directions (e.g.
@@ -451,7 +451,8 @@ def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, ph
     propagation. ``hamiltonian_for(theta_deg, field_mT, phi_deg)`` builds each trajectory's
     Hamiltonian; only its electron part may differ from ``base``. The yield of each trajectory is the
     reference's Simpson yield on ``times_us`` (R/rpm_quantum_needle_jd_sweeps.py:418-420). The map
-    is ``yields.mean(axis=0)``.
+    is ``yields.mean(axis=0)``. ``batch_size="auto"`` picks the batch by timing the first
+    trajectories over a short prefix (``autotune.propagation_batch``).
     """
     from .quadrature import integrate_yield
 
@@ -461,6 +462,12 @@ def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, ph
     fields = np.atleast_1d(np.asarray(fields_uT, dtype=float))
     S, D, F = states.shape[0], thetas.size, fields.size
     combos = [(s, d, f) for s in range(S) for d in range(D) for f in range(F)]
+    if batch_size == "auto":
+        from . import autotune
+        lead = combos[:64]
+        hs = [hamiltonian_for(float(thetas[d]), 1e-3 * float(fields[f]), float(phis[d])) for _, d, f in lead]
+        idx = [s for s, _, _ in lead]
+        batch_size = autotune.propagation_batch(propagator, states[idx], times_us, pair_matrices(base, hs))["batch"]
     out = np.empty((S, D, F))
     for c0 in range(0, len(combos), max(1, int(batch_size))):
         chunk = combos[c0:c0 + max(1, int(batch_size))]
