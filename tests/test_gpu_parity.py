@@ -371,6 +371,24 @@ def test_full_system_short_horizon_matches_oracle():
     assert _rel(a.final_state, b.final_state) < 1e-10
 
 
+def test_random_tensor_full_system_propagation_matches_oracle():
+    """The configuration-1 operator family (14 nuclei with random hyperfine and quadrupole tensors, dim
+    221,184) through the adaptive propagator: same schedule and populations as the oracle."""
+    print("case: random-tensor full system, 0.02 us", flush=True)
+    sysr = workloads.full_random_system(seed=1)
+    h = sysr.hamiltonian(theta_deg=35.0, phi_deg=10.0)
+    o = sysr.hamiltonian(theta_deg=35.0, phi_deg=10.0, builder=ocore.build_hamiltonian)
+    times = np.arange(17) * 0.00125
+    s = core.random_phase_initial_state(h.dimensions, 0, 4)
+    kw = dict(state_error_budget=1e-10, initial_step_us=0.01, maximum_dimension=40)
+    p, op = _propagators(h, o, **kw)
+    a = p.propagate(s, times, engine.SingletPopulationObservable(h.dimensions))
+    b = op.propagate(s, times, oengine.SingletPopulationObservable(o.dimensions))
+    assert [(x[1], x[2], x[3]) for x in a.schedule] == [(x[1], x[2], x[3]) for x in b.schedule]
+    np.testing.assert_allclose(a.observable_values, b.observable_values, atol=1e-10)
+    assert _rel(a.final_state, b.final_state) < 1e-10
+
+
 @pytest.mark.parametrize("kernel", [2, 3])
 @pytest.mark.parametrize("batch", [1, 3, 8, 16])
 def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch, kernel):
