@@ -251,6 +251,43 @@ def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
             "full_horizon_us": cfg.maximum_time_us, "matvecs": matvecs, "rounds": bp.rounds}
 
 
+def directional_probe(h, args, n_dirs=64, fields_uT=(25.0, 50.0), n_states=2, horizon=0.25) -> dict:
+    """The directional-map composition of the batched workload: random-phase states x Fibonacci-sphere
+    field directions x field strengths, through batch.directional_yields with the batch size picked by
+    the run-time tuner, over a short horizon with the speed-probe budget."""
+    import numpy as np
+    import torch
+    from rpm_amd import batch, core, workloads
+    cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+    times = np.arange(int(round(horizon / 0.00125)) + 1) * 0.00125
+    th, ph = workloads.fibonacci_sphere(n_dirs)
+    states = torch.from_numpy(np.stack([core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i)
+                                        for i in range(n_states)])).cuda()
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(h, cfg, probe_horizon=float(times[-1])))
+    build = lambda t, f_mT, p: core.build_hamiltonian(t, f_mT, phi_deg=p)  # noqa: E731
+    batch.directional_yields(h, build, bp, states[:1], th[:2], ph[:2], fields_uT[:1], times[:3], cfg.lifetime_us,
+                             batch_size=2)                                   # warm-up
+    t0 = time.perf_counter()
+    hs = [build(float(t), 1e-3 * f, float(p)) for t, p in zip(th, ph) for f in fields_uT]
+    batch.pair_matrices(h, hs)
+    t_host = time.perf_counter() - t0                                          # host-side Hamiltonians alone
+    del hs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    Y = batch.directional_yields(h, build, bp, states, th, ph, fields_uT, times, cfg.lifetime_us, batch_size="auto")
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    from rpm_amd import autotune
+    chosen = [v["batch"] for v in autotune.cached("batch")]
+    n = Y.size
+    return {"mode": "directional", "trajectories": n, "states": n_states, "directions": n_dirs,
+            "fields_uT": list(fields_uT), "horizon_us": float(times[-1]), "seconds": dt,
+            "host_hamiltonian_seconds": t_host, "batch": chosen[-1] if chosen else None,
+            "trajectory_us_per_s": n * float(times[-1]) / dt,
+            "yields_per_s_equiv": n * float(times[-1]) / dt / cfg.maximum_time_us,
+            "map_mean": float(Y.mean()), "map_range": [float(Y.mean(axis=0).min()), float(Y.mean(axis=0).max())]}
+
+
 def grad_probe(h, args, B=None, horizon=None) -> dict:
     """Yield-only vs yield + adjoint gradient for a batch of production trajectories over a short horizon."""
     import numpy as np
@@ -391,7 +428,8 @@ def main() -> int:
                     help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
-    ap.add_argument("--mode", default="hpsi", choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol"),
+    ap.add_argument("--mode", default="hpsi",
+                    choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -424,6 +462,9 @@ def main() -> int:
         return 0
     if args.mode == "sobol":
         print(json.dumps(sobol_probe(args)), flush=True)
+        return 0
+    if args.mode == "directional":
+        print(json.dumps(directional_probe(h, args)), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()
@@ -532,6 +573,8 @@ def main() -> int:
         torch.cuda.empty_cache()
         print("[bench] batched-trajectory probe", file=sys.stderr, flush=True)
         result["batched"] = batch_probe(h, args, B=32, horizon=0.25, plain_sample=4)
+        print("[bench] directional-map probe", file=sys.stderr, flush=True)
+        result["directional"] = directional_probe(h, args)
         print("[bench] gradient probe", file=sys.stderr, flush=True)
         result["gradient"] = grad_probe(h, args, B=8, horizon=0.05)
         print("[bench] factorized spectral probe", file=sys.stderr, flush=True)

@@ -35,6 +35,12 @@ def clear_cache() -> None:
         _CACHE.clear()
 
 
+def cached(kind: str) -> list:
+    """The cached results of one tuner ("hpsi" or "batch"), oldest first."""
+    with _LOCK:
+        return [v for k, v in _CACHE.items() if k[0] == kind]
+
+
 def hpsi_kernel(hamiltonian, batch: int, *, reps: int = 5, apply: bool = True) -> dict:
     """Time every H·psi kernel the operator supports on `batch` resident vectors (HIP events) and pick
     the fastest. Returns {"kernel", "name", "ms_per_launch": {kernel: ms}}; with `apply` the operator

@@ -462,17 +462,27 @@ def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, ph
     fields = np.atleast_1d(np.asarray(fields_uT, dtype=float))
     S, D, F = states.shape[0], thetas.size, fields.size
     combos = [(s, d, f) for s in range(S) for d in range(D) for f in range(F)]
+    cache: dict = {}
+
+    def electron_matrices(sel):
+        """4x4 electron matrices of (direction, strength) pairs, each Hamiltonian built once."""
+        todo = [(d, f) for _, d, f in sel if (d, f) not in cache]
+        todo = list(dict.fromkeys(todo))
+        if todo:
+            hs = [hamiltonian_for(float(thetas[d]), 1e-3 * float(fields[f]), float(phis[d])) for d, f in todo]
+            for key, mat in zip(todo, pair_matrices(base, hs)):
+                cache[key] = mat
+        return np.stack([cache[(d, f)] for _, d, f in sel])
+
     if batch_size == "auto":
         from . import autotune
         lead = combos[:64]
-        hs = [hamiltonian_for(float(thetas[d]), 1e-3 * float(fields[f]), float(phis[d])) for _, d, f in lead]
         idx = [s for s, _, _ in lead]
-        batch_size = autotune.propagation_batch(propagator, states[idx], times_us, pair_matrices(base, hs))["batch"]
+        batch_size = autotune.propagation_batch(propagator, states[idx], times_us, electron_matrices(lead))["batch"]
     out = np.empty((S, D, F))
     for c0 in range(0, len(combos), max(1, int(batch_size))):
         chunk = combos[c0:c0 + max(1, int(batch_size))]
-        hs = [hamiltonian_for(float(thetas[d]), 1e-3 * float(fields[f]), float(phis[d])) for _, d, f in chunk]
-        pairs = pair_matrices(base, hs)
+        pairs = electron_matrices(chunk)
         idx = [s for s, _, _ in chunk]
         x0 = states[idx] if not nat.is_torch(states) else states[idx]
         results = propagator.propagate(x0, times_us, pairs=pairs)
