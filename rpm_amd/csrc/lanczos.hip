@@ -514,6 +514,8 @@ __global__ void __launch_bounds__(256) k_gram_reduce(const c128* __restrict__ pa
   const int a = e / k, c = e % k;
   const c128* pb = partial + static_cast<long long>(bt) * nblk * kGramTile * kGramTile;
   c128 x{0.0, 0.0}, y{0.0, 0.0};
+  // unrolled so that the loads run ahead of the (ordered) additions
+#pragma unroll 8
   for (int i = 0; i < nblk; ++i) {
     x = cadd(x, pb[static_cast<long long>(i) * kGramTile * kGramTile + a * kGramTile + c]);
     y = cadd(y, pb[static_cast<long long>(i) * kGramTile * kGramTile + c * kGramTile + a]);
