@@ -133,14 +133,25 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
   }
 }
 
-// h[c] = sum_blk partial[c][blk]; optionally alpha[j] = Re h[alpha_col].
+// h[c] = sum_blk partial[c][blk]; optionally alpha[j] = Re h[alpha_col]. With dpart, the last block
+// of each trajectory instead forms dsum[b] = sum_blk dpart[b][blk] (the |y|^2 of the lagged pass),
+// in the order k_lcgs_finish used to.
 __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict__ partial, int nblk, long long pstride,
                                                           c128* __restrict__ h, int hstride, double* __restrict__ alpha,
                                                           int astride, int alpha_col, int j, int need,
-                                                          const BasisCtl* ctl) {
+                                                          const BasisCtl* ctl, const double* __restrict__ dpart,
+                                                          double* __restrict__ dsum) {
   const int b = blockIdx.y;
   if (!basis_live(ctl, b, j, need)) return;
   __shared__ c128 sm[kThreads / kWave];
+  if (dpart && blockIdx.x == gridDim.x - 1) {
+    __shared__ double smd[kThreads / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nblk; i += kThreads) s += dpart[static_cast<long long>(b) * nblk + i];
+    s = block_sum(s, smd);
+    if (threadIdx.x == 0) dsum[b] = s;
+    return;
+  }
   const int c = blockIdx.x;
   const c128* p = partial + b * pstride + static_cast<long long>(c) * nblk;
   c128 s{0.0, 0.0};
@@ -315,7 +326,7 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
 // block of the trajectory to finish (device-scope counter; the fences run on this rare path only)
 // takes beta_j from the corrected vector and clears c_{j+1}.
 __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict__ V, long long vstride, long long n,
-                                                          const double* __restrict__ ypart, int nblk,
+                                                          const double* __restrict__ ysum,
                                                           const c128* __restrict__ h, int hstride,
                                                           c128* __restrict__ cpend, int cstride,
                                                           const double* __restrict__ atilde, double* __restrict__ alpha,
@@ -331,8 +342,8 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   const c128* hb = h + b * hstride;
   c128* cb = cpend + b * cstride;
   BasisCtl& C = ctl[b];
-  double yy = 0.0, gg = 0.0;
-  for (int i = t; i < nblk; i += kThreads) yy += ypart[static_cast<long long>(b) * nblk + i];
+  const double yy = ysum[b];
+  double gg = 0.0;
   if (t < j) {
     const c128 v = hb[t];
     gg = v.re * v.re + v.im * v.im;
@@ -343,8 +354,7 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   const double cprev = j > 0 ? cb[j - 1].re : 0.0;
   const double bprev = C.beta_prev;
   const int target = C.target;
-  yy = block_sum(yy, sm);   // ends with a barrier: every read above precedes the writes below
-  gg = block_sum(gg, sm);
+  gg = block_sum(gg, sm);   // ends with a barrier: every read above precedes the writes below
   const c128 glast = cmake(gam.re - at, gam.im);
   gg += glast.re * glast.re + glast.im * glast.im;
   const bool lead = blockIdx.x == 0;
@@ -581,6 +591,7 @@ struct BasisWs {
   c128* cpend;       // [batch][m] pending correction c_{j+1}
   double* fpart;     // [batch][kFixBlocks]
   double* atilde;    // [batch] alpha~_j
+  double* ysum;      // [batch] |y|^2
   unsigned* counter; // [batch]
   double* sq;        // [batch][nchunk]
   BasisCtl* ctl;     // [batch]
@@ -600,7 +611,7 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   const size_t o_w0 = take(B * n * 16), o_w1 = take(B * n * 16);
   const size_t o_p = take(B * m * npass * 16), o_yp = take(B * npass * 8), o_h = take(B * m * 16);
   const size_t o_c = take(B * m * 16), o_fp = take(B * kFixBlocks * 8), o_cnt = take(B * sizeof(unsigned));
-  const size_t o_at = take(B * 8);
+  const size_t o_at = take(B * 8), o_ys = take(B * 8);
   const size_t o_sq = take(B * nchunk * 8), o_ctl = take(B * sizeof(BasisCtl));
   const size_t o_dots = take(B * static_cast<size_t>(units) * 16);
   BasisWs w;
@@ -614,6 +625,7 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   w.cpend = reinterpret_cast<c128*>(base + o_c);
   w.fpart = reinterpret_cast<double*>(base + o_fp);
   w.atilde = reinterpret_cast<double*>(base + o_at);
+  w.ysum = reinterpret_cast<double*>(base + o_ys);
   w.counter = reinterpret_cast<unsigned*>(base + o_cnt);
   w.sq = reinterpret_cast<double*>(base + o_sq);
   w.ctl = reinterpret_cast<BasisCtl*>(base + o_ctl);
@@ -682,13 +694,12 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     else
       launch_pass<16>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(j + 1, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
-                       nullptr, m, 0, j, 0, w.ctl);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(j + 2, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
+                       nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
     RPM_LAUNCH_CHECK();
     // alpha_j, beta_j, c_{j+1}; the explicit correction when it is due
     hipLaunchKernelGGL(k_lcgs_finish, dim3(j == count - 1 ? 1 : kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n,
-                       w.ypart, w.npass, w.h, m, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter,
-                       w.ctl);
+                       w.ysum, w.h, m, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter, w.ctl);
     RPM_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_basis_info, dim3(ceil_div(batch, 64)), dim3(64), 0, s, w.ctl, info, batch);
