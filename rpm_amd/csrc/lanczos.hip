@@ -513,24 +513,37 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long l
     for (int v = 0; v < 4; ++v) out[(ia * 4 + u) * kGramTile + ib * 4 + v] = acc[u][v];
 }
 
-// P_b = Hermitian part of sum over blocks of the partial tiles (k x k row-major at P + b*pstride).
-__global__ void __launch_bounds__(256) k_gram_reduce(const c128* __restrict__ partial, int nblk,
-                                                     const int* __restrict__ kb, int k1, c128* __restrict__ P,
-                                                     long long pstride) {
+// P_b = Hermitian part of the sum over blocks of the partial tiles (k x k row-major at P + b*pstride),
+// in two steps so that every read is coalesced: k_gram_sum adds the tiles (each entry over the blocks
+// in ascending order) into P, k_gram_sym replaces each pair (a, c), (c, a) by the Hermitian part.
+__global__ void __launch_bounds__(256) k_gram_sum(const c128* __restrict__ partial, int nblk,
+                                                  const int* __restrict__ kb, int k1, c128* __restrict__ P,
+                                                  long long pstride) {
   const int bt = blockIdx.y;
   const int k = kb ? kb[bt] : k1;
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= k * k) return;
   const int a = e / k, c = e % k;
-  const c128* pb = partial + static_cast<long long>(bt) * nblk * kGramTile * kGramTile;
-  c128 x{0.0, 0.0}, y{0.0, 0.0};
+  const c128* pb = partial + static_cast<long long>(bt) * nblk * kGramTile * kGramTile + a * kGramTile + c;
+  c128 x{0.0, 0.0};
   // unrolled so that the loads run ahead of the (ordered) additions
 #pragma unroll 8
-  for (int i = 0; i < nblk; ++i) {
-    x = cadd(x, pb[static_cast<long long>(i) * kGramTile * kGramTile + a * kGramTile + c]);
-    y = cadd(y, pb[static_cast<long long>(i) * kGramTile * kGramTile + c * kGramTile + a]);
-  }
-  P[bt * pstride + e] = cmake(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+  for (int i = 0; i < nblk; ++i) x = cadd(x, pb[static_cast<long long>(i) * kGramTile * kGramTile]);
+  P[bt * pstride + e] = x;
+}
+
+__global__ void __launch_bounds__(256) k_gram_sym(const int* __restrict__ kb, int k1, c128* __restrict__ P,
+                                                  long long pstride) {
+  const int bt = blockIdx.y;
+  const int k = kb ? kb[bt] : k1;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= k * k) return;
+  const int a = e / k, c = e % k;
+  if (a > c) return;
+  c128* pb = P + bt * pstride;
+  const c128 x = pb[a * k + c], y = pb[c * k + a];
+  pb[a * k + c] = cmake(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
+  pb[c * k + a] = cmake(0.5 * (y.re + x.re), 0.5 * (y.im - x.im));
 }
 
 }  // namespace rpm
@@ -803,8 +816,10 @@ int gram_batch(const c128* V, long long vstride, const int* d_k, int k1, int bat
   c128* partial = static_cast<c128*>(ws);
   hipLaunchKernelGGL(k_gram, dim3(nblk, batch), dim3(256), 0, s, V, vstride, d_k, k1, nn, per, partial);
   RPM_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_gram_reduce, dim3(kGramTile * kGramTile / 256, batch), dim3(256), 0, s, partial, nblk, d_k, k1,
-                     P, pstride);
+  hipLaunchKernelGGL(k_gram_sum, dim3(kGramTile * kGramTile / 256, batch), dim3(256), 0, s, partial, nblk, d_k, k1, P,
+                     pstride);
+  RPM_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_gram_sym, dim3(kGramTile * kGramTile / 256, batch), dim3(256), 0, s, d_k, k1, P, pstride);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }
