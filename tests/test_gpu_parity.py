@@ -18,7 +18,6 @@ torch = pytest.importorskip("torch")
 from oracle import core as ocore  # noqa: E402
 from oracle import dense as odense  # noqa: E402
 from oracle import engine as oengine  # noqa: E402
-from oracle import model as omodel  # noqa: E402
 from rpm_amd import _native as nat  # noqa: E402
 from rpm_amd import core, engine, model, workloads  # noqa: E402
 
