@@ -208,15 +208,22 @@ constexpr double kFixThreshold = 1e-6;     // |c_{j+1}| above which the correcti
 constexpr double kCancel = 1e-2;           // beta^2 <= kCancel |y|^2: the norm is taken explicitly
 constexpr int kFixBlocks = 256;            // blocks per trajectory of k_lcgs_finish
 constexpr int kPassWaves = 4;
+// columns per wave and round, rounds: for j <= 32 and j <= 64. Measured (32 trajectories, batched
+// trajectory-us/s): one round of 8 or 16 columns 20.8, rounds of 8 22.3, of 4 22.9, of 2 23.1.
+constexpr int kRound32[2] = {2, 4};
+constexpr int kRound64[2] = {2, 8};
 static_assert(kPassWaves * kWave == kThreads, "k_lcgs_pass shares the kThreads-wide reductions");
 
-// Columns c < j are dealt to the waves: column c to wave c % 4, CPW per wave. Whatever CPW, a
-// column's partial is formed by the same lanes over the same elements in the same order, and the
-// per-element correction is summed over the waves in a fixed order, so the basis is bitwise the
-// same for every CPW and batch size. kStream: the basis columns are read with non-temporal loads.
+// Columns c < j are dealt to the waves: column c to wave c % 4, CPR per wave and round, ROUNDS
+// rounds (columns q + 4 (round CPR + i)). Whatever the split, a column's partial is formed by the
+// same lanes over the same elements in the same order, and each element's correction adds the
+// wave's columns in ascending order (carried in LDS from one round to the next) and then the waves
+// in a fixed order, so the basis is bitwise the same for every split and batch size. Rounds keep
+// the accumulators of few columns in registers at a time (higher occupancy). kStream: the basis
+// columns are read with non-temporal loads.
 // That pays when the bases of a batch are far larger than the 256 MB Infinity Cache (+5% at 32
 // trajectories) and costs when one trajectory's basis fits in it (-10%).
-template <int CPW, bool kStream>
+template <int CPR, int ROUNDS, bool kStream>
 __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, long long vstride, long long n, int j,
                                                         const c128* __restrict__ in, long long ild,
                                                         const c128* __restrict__ cpend, int cstride,
@@ -259,40 +266,43 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
   if (threadIdx.x < j) s_c[threadIdx.x] = cpend[b * cstride + threadIdx.x];
   __syncthreads();
   const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
-  c128 acc[CPW];
+  const int nblk = gridDim.x;
+#pragma unroll 1
+  for (int rd = 0; rd < ROUNDS; ++rd) {
+    c128 acc[CPR];
 #pragma unroll
-  for (int i = 0; i < CPW; ++i) acc[i] = cmake(0.0, 0.0);
-  for (int it = 0; it < kPassPer; ++it) {
-    const int e = it * kWave + lane;
-    c128 corr = cmake(0.0, 0.0);
-    if (e < cnt) {
-      const c128 uu = s_u[e];
+    for (int i = 0; i < CPR; ++i) acc[i] = cmake(0.0, 0.0);
+    for (int it = 0; it < kPassPer; ++it) {
+      const int e = it * kWave + lane;
+      c128 corr = rd == 0 ? cmake(0.0, 0.0) : s_corr[q][e];   // this lane's own running sum
+      if (e < cnt) {
+        const c128 uu = s_u[e];
 #pragma unroll
-      for (int i = 0; i < CPW; ++i) {
-        const int c = q + kPassWaves * i;
-        if (c < j) {
-          const c128* pv = Vb + static_cast<long long>(c) * n + x0 + e;
-          c128 v;
-          if constexpr (kStream) {
-            v.re = __builtin_nontemporal_load(&pv->re);
-            v.im = __builtin_nontemporal_load(&pv->im);
-          } else {
-            v = *pv;
+        for (int i = 0; i < CPR; ++i) {
+          const int c = q + kPassWaves * (rd * CPR + i);
+          if (c < j) {
+            const c128* pv = Vb + static_cast<long long>(c) * n + x0 + e;
+            c128 v;
+            if constexpr (kStream) {
+              v.re = __builtin_nontemporal_load(&pv->re);
+              v.im = __builtin_nontemporal_load(&pv->im);
+            } else {
+              v = *pv;
+            }
+            acc[i] = cfmaconj(v, uu, acc[i]);
+            corr = cfma(v, s_c[c], corr);
           }
-          acc[i] = cfmaconj(v, uu, acc[i]);
-          corr = cfma(v, s_c[c], corr);
         }
       }
+      s_corr[q][e] = corr;
     }
-    s_corr[q][e] = corr;
-  }
-  const int nblk = gridDim.x;
 #pragma unroll
-  for (int i = 0; i < CPW; ++i) {
-    const int c = q + kPassWaves * i;
-    if (c < j) {
-      const c128 t = wave_sum(acc[i]);
-      if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
+    for (int i = 0; i < CPR; ++i) {
+      const int c = q + kPassWaves * (rd * CPR + i);
+      if (c < j) {
+        const c128 t = wave_sum(acc[i]);
+        if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
+      }
     }
   }
   __syncthreads();
@@ -648,17 +658,17 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   return w;
 }
 
-template <int CPW>
+template <int CPR, int ROUNDS>
 void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int j, const c128* in, long long ild, int m,
                  bool fuse_alpha, c128* out, int batch, hipStream_t s) {
   if (batch > 1)
-    hipLaunchKernelGGL((k_lcgs_pass<CPW, true>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j, in, ild,
-                       w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride,
-                       w.ypart, w.ctl);
+    hipLaunchKernelGGL((k_lcgs_pass<CPR, ROUNDS, true>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j,
+                       in, ild, w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
+                       w.pstride, w.ypart, w.ctl);
   else
-    hipLaunchKernelGGL((k_lcgs_pass<CPW, false>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j, in, ild,
-                       w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial, w.pstride,
-                       w.ypart, w.ctl);
+    hipLaunchKernelGGL((k_lcgs_pass<CPR, ROUNDS, false>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j,
+                       in, ild, w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
+                       w.pstride, w.ypart, w.ctl);
 }
 
 // Shared implementation of the single and batched basis builders.
@@ -701,11 +711,11 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     }
     // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
     if (j <= kPassWaves * 4)
-      launch_pass<4>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
+      launch_pass<4, 1>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else if (j <= kPassWaves * 8)
-      launch_pass<8>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
+      launch_pass<kRound32[0], kRound32[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else
-      launch_pass<16>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
+      launch_pass<kRound64[0], kRound64[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(j + 2, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
                        nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
