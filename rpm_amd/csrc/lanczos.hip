@@ -208,8 +208,10 @@ constexpr double kFixThreshold = 1e-6;     // |c_{j+1}| above which the correcti
 constexpr double kCancel = 1e-2;           // beta^2 <= kCancel |y|^2: the norm is taken explicitly
 constexpr int kFixBlocks = 256;            // blocks per trajectory of k_lcgs_finish
 constexpr int kPassWaves = 4;
-// columns per wave and round, rounds: for j <= 32 and j <= 64. Measured (32 trajectories, batched
-// trajectory-us/s): one round of 8 or 16 columns 20.8, rounds of 8 22.3, of 4 22.9, of 2 23.1.
+// columns per wave and round, rounds: for j <= 16, j <= 32 and j <= 64. Measured (32 trajectories,
+// batched trajectory-us/s): one round of 8 or 16 columns 20.8, rounds of 8 22.3, of 4 22.9, of 2
+// 23.1; rounds of 2 also for j <= 16 (one round of 4 before) 23.15.
+constexpr int kRound16[2] = {2, 2};
 constexpr int kRound32[2] = {2, 4};
 constexpr int kRound64[2] = {2, 8};
 static_assert(kPassWaves * kWave == kThreads, "k_lcgs_pass shares the kThreads-wide reductions");
@@ -711,7 +713,7 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     }
     // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
     if (j <= kPassWaves * 4)
-      launch_pass<4, 1>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
+      launch_pass<kRound16[0], kRound16[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else if (j <= kPassWaves * 8)
       launch_pass<kRound32[0], kRound32[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     else
