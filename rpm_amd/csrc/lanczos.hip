@@ -4,7 +4,7 @@
 //
 // Orthogonalisation of w = H v_j (two projections, as classical Gram-Schmidt
 // applied twice): the three-term step w - beta_{j-1} v_{j-1} - alpha_j v_j
-// (beta subtracted inside the matvec) and then one full projection
+// (formed inside k_lcgs_pass) and then one full projection
 // w <- w - V (V^H w), applied one step late so that the projection of step j
 // and the materialisation of v_{j+1} share one read of V (k_lcgs_pass).  The
 // reference subtracts the same three terms and then projects twice more; the
@@ -210,7 +210,7 @@ constexpr int kFixBlocks = 256;            // blocks per trajectory of k_lcgs_fi
 constexpr int kPassWaves = 4;
 // columns per wave and round, rounds: for j <= 16, j <= 32 and j <= 64. Measured (32 trajectories,
 // batched trajectory-us/s): one round of 8 or 16 columns 20.8, rounds of 8 22.3, of 4 22.9, of 2
-// 23.1; rounds of 2 also for j <= 16 (one round of 4 before) 23.15.
+// 23.1; for j <= 16, rounds of 2 gave 23.15 against 23.0 for one round of 4.
 constexpr int kRound16[2] = {2, 2};
 constexpr int kRound32[2] = {2, 4};
 constexpr int kRound64[2] = {2, 8};
