@@ -23,7 +23,7 @@ from .model import (
     ALL_NUCLEI, GAMMA_E_MHZ_PER_MT, NUCLEAR_RADICALS, QUADRUPOLE_TENSORS_MT, TWOPI, field_vector_mT,
     spin_operators,
 )
-from .quadrature import integrate_yield, simpson_weights, yield_weights  # noqa: F401  (re-exported)
+from .quadrature import integrate_singlet_yield, integrate_yield, simpson_weights, yield_weights  # noqa: F401  (re-exported)
 
 MASTER_SEED = 20260731
 R2_D_PAPER_UT = -2783.0 / (2.0 ** 3)

@@ -59,3 +59,33 @@ def yield_weights(times_us, lifetime_us) -> np.ndarray:
 def integrate_yield(times_us, populations, lifetime_us) -> float:
     """Phi_S for one population trace (R/rpm_quantum_needle_jd_sweeps.py:418-420)."""
     return float(np.dot(yield_weights(times_us, lifetime_us), np.asarray(populations, dtype=float)))
+
+
+def trapezoid_weights(x) -> np.ndarray:
+    """Weights of the composite trapezoid rule on the nodes x: half of each adjacent spacing."""
+    x = np.asarray(x, dtype=float)
+    w = np.zeros(x.size)
+    if x.size < 2:
+        return w
+    h = np.diff(x)
+    w[:-1] += 0.5 * h
+    w[1:] += 0.5 * h
+    return w
+
+
+def integrate_singlet_yield(times_us, singlet_population_values, lifetime_us, *, method: str = "simpson") -> float:
+    """Phi_S of an already-propagated trajectory (R/rpm_quantum_needle_validation.py:192-210):
+    the integral of exp(-t/tau) P_S(t) / tau by Simpson's rule (as ``integrate_yield``) or by the
+    trapezoid rule. ValueError when the times and populations are not matching 1-D arrays, or for
+    another method."""
+    times = np.asarray(times_us, dtype=float)
+    populations = np.asarray(singlet_population_values, dtype=float)
+    if times.shape != populations.shape or times.ndim != 1:
+        raise ValueError("Yield times and singlet populations must be matching 1-D arrays.")
+    if method == "simpson":
+        weights = simpson_weights(times)
+    elif method == "trapezoid":
+        weights = trapezoid_weights(times)
+    else:
+        raise ValueError(f"Unknown yield-integration method: {method}")
+    return float(np.dot(weights * (np.exp(-times / lifetime_us) / lifetime_us), populations))

@@ -202,6 +202,30 @@ def test_simpson_weights_match_scipy(x):
     assert abs(np.dot(quadrature.simpson_weights(x), y) - scipy.integrate.simpson(y, x=x)) < 1e-14
 
 
+@pytest.mark.parametrize("x", [
+    np.linspace(0.0, 1.0, 5), np.linspace(0.0, 1.0, 6), np.array([0.0, 0.3]),
+    np.array([0.0, 0.1, 0.25, 0.3, 0.7, 0.71, 1.0]),
+])
+def test_integrate_singlet_yield_rules(x):
+    """R/rpm_quantum_needle_validation.py:192-210: both rules against the library routines."""
+    rng = np.random.default_rng(2)
+    p = rng.uniform(size=x.size)
+    f = np.exp(-x / 0.7) * p / 0.7
+    assert abs(pcore.integrate_singlet_yield(x, p, 0.7) - scipy.integrate.simpson(f, x=x)) < 1e-14
+    assert abs(pcore.integrate_singlet_yield(x, p, 0.7, method="trapezoid") - np.trapezoid(f, x)) < 1e-14
+    assert pcore.integrate_singlet_yield(x, p, 0.7) == pcore.integrate_yield(x, p, 0.7)
+
+
+def test_integrate_singlet_yield_errors():
+    t = np.linspace(0.0, 1.0, 5)
+    with pytest.raises(ValueError, match="matching 1-D"):
+        pcore.integrate_singlet_yield(t, np.ones(4), 1.0)
+    with pytest.raises(ValueError, match="matching 1-D"):
+        pcore.integrate_singlet_yield(t[None, :], np.ones((1, 5)), 1.0)
+    with pytest.raises(ValueError, match="Unknown yield-integration method"):
+        pcore.integrate_singlet_yield(t, np.ones(5), 1.0, method="midpoint")
+
+
 def test_production_grid_yield_weights():
     cfg = pcore.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
     t = cfg.time_grid_us
