@@ -10,11 +10,12 @@ probes: "batched" (32 trajectories = 8 field orientations x 4 random-phase
 vectors through the reference's adaptive Lanczos rules, against the plain
 one-trajectory-at-a-time path) and "gradient" (yield plus adjoint gradient
 over yield alone), plus "spectral" (the factorized J = D = 0 oracle's 199-angle
-curve on the GPU against the oracle's per-orientation CPU time) and "sobol" (16 batched
-Sobol reaction-time cells on the full system).
+curve on the GPU against the oracle's per-orientation CPU time), "sobol" (16 batched
+Sobol reaction-time cells on the full system) and "trajectory" (one whole 57.56 us
+trajectory of the reference's archived run, against its archived wall time).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
-    python bench.py --mode propagate|batch|grad|spectral|sobol|copy ...
+    python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
 """
 
 from __future__ import annotations
@@ -192,6 +193,55 @@ def propagate_probe(h, args, horizon=None) -> dict:
                          "definition": "reference CGS2 semantics over the recorded schedule",
                          "device_bytes": dbytes, "device_achieved": dbytes / dt / 1e9,
                          "device_frac": dbytes / dt / 1e9 / HBM_PEAK_GBS}}
+
+
+# Wall time per full-system trajectory of the reference's archived uncoupled run (n = 232, one
+# numerical thread per process): results/quadrupole/full14_uncoupled_tau5_29angles/
+# execution_result.json, durations_seconds.
+REFERENCE_TRAJECTORY_SECONDS = {"median": 25978.29, "min": 12351.87, "max": 28132.45, "n": 232, "threads": 1}
+ARCHIVED_YIELDS = os.path.join(ROOT, "tests", "golden", "archived_full14_yields.csv")
+
+
+def trajectory_probe(theta: float = 90.0, quadrupole_scale: float = 1.0, sample: int = 0) -> dict:
+    """One whole trajectory of the reference's archived uncoupled run, end to end: D = J = 0,
+    14N quadrupoles on, 50 uT at theta, tau = 5 us, tail bound 1e-5 (horizon 57.5646 us on the
+    46,053-point grid), budget 2.4e-5, seed 20260731, sample `sample`
+    (R/rpm_full14_uncoupled_quadrupole.py, R/rpm_full14_quadrupole_experiment.py:154-194).
+    Timed from the host initial state to the host yield, as the reference's per-trajectory wall
+    time is; the archived yield of the same cell is read from the repository's fixture."""
+    import csv
+
+    import numpy as np
+    import torch
+    from rpm_amd import core
+    model = core.ProductionModel(quadrupole_scale=quadrupole_scale, ratio_D_over_J=None)
+    cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
+                           state_error_budget=2.4e-5, quadrupole_scale=quadrupole_scale)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    h = core.build_hamiltonian(theta, 0.05, model=model)
+    state = core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, sample)
+    out = core.propagate_trajectory(h, state, cfg)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    st = out["result"].stats
+    archived = None
+    if os.path.exists(ARCHIVED_YIELDS):
+        with open(ARCHIVED_YIELDS) as f:
+            for r in csv.DictReader(f):
+                if (r["electron_interactions"] == "D=J=0" and float(r["scale"]) == quadrupole_scale
+                        and float(r["theta_deg"]) == theta and int(r["sample_id"]) == sample):
+                    archived = float(r["yield"])
+    horizon = float(out["times_us"][-1])
+    ref = REFERENCE_TRAJECTORY_SECONDS
+    return {"mode": "trajectory", "cell": {"electron_interactions": "D=J=0", "quadrupole_scale": quadrupole_scale,
+                                            "theta_deg": theta, "field_uT": 50.0, "tau_us": 5.0, "sample_id": sample},
+            "horizon_us": horizon, "grid_points": int(out["times_us"].size), "seconds": dt,
+            "simulated_us_per_s": horizon / dt, "accepted": st.accepted_steps, "rejected": st.rejected_steps,
+            "matvecs": st.matvecs, "yield": out["yield"], "archived_yield": archived,
+            "yield_minus_archived": None if archived is None else out["yield"] - archived,
+            "yield_propagation_estimate": out["yield_propagation_estimate"],
+            "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / dt}
 
 
 def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
@@ -429,7 +479,8 @@ def main() -> int:
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi",
-                    choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional"),
+                    choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
+                             "trajectory"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -465,6 +516,9 @@ def main() -> int:
         return 0
     if args.mode == "directional":
         print(json.dumps(directional_probe(h, args)), flush=True)
+        return 0
+    if args.mode == "trajectory":
+        print(json.dumps(trajectory_probe()), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()      # the H·ψ measurement only
@@ -583,6 +637,8 @@ def main() -> int:
         result["sobol"] = sobol_probe(args)
         print("[bench] propagation probe", file=sys.stderr, flush=True)
         result["propagation"] = propagate_probe(h, args, horizon=0.1)
+        print("[bench] full-trajectory probe", file=sys.stderr, flush=True)
+        result["trajectory"] = trajectory_probe()
     if not args.no_cpu and not args.no_extras:
         print("[bench] CPU trajectory and all-core baselines", file=sys.stderr, flush=True)
         extra = cpu_extra_baselines()
