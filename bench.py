@@ -16,6 +16,7 @@ trajectory of the reference's archived run, against its archived wall time).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
     python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
+    python bench.py --mode archive_batch   # 32 whole archived trajectories in one batch (about a minute)
 """
 
 from __future__ import annotations
@@ -242,6 +243,58 @@ def trajectory_probe(theta: float = 90.0, quadrupole_scale: float = 1.0, sample:
             "yield_minus_archived": None if archived is None else out["yield"] - archived,
             "yield_propagation_estimate": out["yield_propagation_estimate"],
             "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / dt}
+
+
+def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0), samples: int = 4,
+                        quadrupole_scale: float = 1.0) -> dict:
+    """32 whole trajectories of the reference's archived uncoupled run (8 field angles x samples
+    0-3, D = J = 0, quadrupoles on, the settings of trajectory_probe) in one batched propagation,
+    each yield compared with its archived value. Timed from the host states to the host yields."""
+    import csv
+
+    import numpy as np
+    import torch
+    from rpm_amd import batch, core
+    model = core.ProductionModel(quadrupole_scale=quadrupole_scale, ratio_D_over_J=None)
+    cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
+                           state_error_budget=2.4e-5, quadrupole_scale=quadrupole_scale)
+    archived = {}
+    with open(ARCHIVED_YIELDS) as f:
+        for r in csv.DictReader(f):
+            if r["electron_interactions"] == "D=J=0" and float(r["scale"]) == quadrupole_scale:
+                archived[(float(r["theta_deg"]), int(r["sample_id"]))] = float(r["yield"])
+    cells = [(float(t), s) for t in thetas for s in range(samples)]
+    missing = [c for c in cells if c not in archived]
+    if missing:
+        raise ValueError(f"cells not in the archived run: {missing}")
+    times = cfg.time_grid_us
+
+    def progress(b, now, accepted, rejected, matvecs, acc_err):
+        if b == 0 and accepted % 200 == 0:
+            print(f"[bench] archive batch: trajectory 0 at {now:.2f} us, {accepted} steps", file=sys.stderr,
+                  flush=True)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    base = core.build_hamiltonian(90.0, 0.05, model=model)
+    hs = [core.build_hamiltonian(t, 0.05, model=model) for t in thetas]
+    pairs = batch.pair_matrices(base, [hs[i // samples] for i in range(len(cells))])
+    states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, s) for _, s in cells])
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
+    res = bp.propagate(states, times, pairs=pairs, progress_callback=progress)
+    yields = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    diff = yields - np.array([archived[c] for c in cells])
+    horizon = float(times[-1])
+    per_traj = dt / len(cells)
+    ref = REFERENCE_TRAJECTORY_SECONDS
+    return {"mode": "archive_batch", "trajectories": len(cells), "thetas_deg": list(thetas), "samples": samples,
+            "quadrupole_scale": quadrupole_scale, "horizon_us": horizon, "seconds": dt,
+            "trajectory_us_per_s": len(cells) * horizon / dt, "seconds_per_trajectory": per_traj,
+            "matvecs": int(sum(r.stats.matvecs for r in res)), "rounds": bp.rounds,
+            "max_abs_yield_minus_archived": float(np.abs(diff).max()),
+            "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / per_traj}
 
 
 def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
@@ -480,7 +533,7 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
-                             "trajectory"),
+                             "trajectory", "archive_batch"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -519,6 +572,9 @@ def main() -> int:
         return 0
     if args.mode == "trajectory":
         print(json.dumps(trajectory_probe()), flush=True)
+        return 0
+    if args.mode == "archive_batch":
+        print(json.dumps(archive_batch_probe()), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()      # the H·ψ measurement only

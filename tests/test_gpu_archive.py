@@ -5,9 +5,10 @@ reference (fixture copy: tests/golden/archived_full14_yields.csv) holds 464
 per-trajectory singlet yields at dim 221,184 (50 uT, tau = 5 us, tail bound
 1e-5, horizon 57.5646 us, 46,053-point grid, production Krylov rules, budget
 2.4e-5, seed 20260731).  Each archived row took 3.4-7.8 CPU hours.  Two rows
-are re-run here from scratch on the GPU; the yield must agree within the
-archive's own propagation estimate (~1e-6), and the fine/coarse quadrature
-difference within 1e-9.
+are re-run here from scratch on the GPU, one trajectory at a time; the yield
+must agree within the archive's own propagation estimate (~1e-6), and the
+fine/coarse quadrature difference within 1e-9.  Four more rows run as one
+batched propagation and must agree with the archive to 1e-10.
 """
 
 import csv
@@ -60,3 +61,27 @@ def test_archived_full_system_yield(interactions, scale, theta, sample):
     assert abs(out["yield"] - ref) < max(2e-6, float(row["yield_propagation_estimate"]))
     assert abs(out["quadrature_error"] - float(row["quadrature_error"])) < 1e-9
     assert out["yield_propagation_estimate"] < 5e-6
+
+
+def test_archived_yields_batched():
+    """Four archived cells (D = J = 0, quadrupoles on, four field angles, sample 2) in one batched
+    propagation over the whole horizon: every yield equals the archived one to 1e-10."""
+    from rpm_amd import batch
+    rows = _rows()
+    thetas = (0.0, 60.0, 120.0, 180.0)
+    sample = 2
+    print(f"case: batched archived cells theta={thetas} sample={sample}", flush=True)
+    model = core.ProductionModel(quadrupole_scale=1.0, ratio_D_over_J=None)
+    cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
+                           state_error_budget=2.4e-5, quadrupole_scale=1.0)
+    base = core.build_hamiltonian(90.0, 0.05, model=model)
+    pairs = batch.pair_matrices(base, [core.build_hamiltonian(t, 0.05, model=model) for t in thetas])
+    state = core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, sample)
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
+    times = cfg.time_grid_us
+    res = bp.propagate(np.stack([state] * len(thetas)), times, pairs=pairs)
+    for t, r in zip(thetas, res):
+        ref = float(_pick(rows, "D=J=0", 1.0, t, sample)["yield"])
+        got = core.integrate_yield(times, r.observable_values, cfg.lifetime_us)
+        print(f"theta={t}: batched {got:.15f} archive {ref:.15f} diff {got - ref:.2e}", flush=True)
+        assert abs(got - ref) < 1e-10
