@@ -17,6 +17,7 @@ trajectory of the reference's archived run, against its archived wall time).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
     python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
     python bench.py --mode archive_batch   # 32 whole archived trajectories in one batch (about a minute)
+    python bench.py --mode gradcheck       # whole-horizon adjoint gradient against central differences
 """
 
 from __future__ import annotations
@@ -424,6 +425,47 @@ def grad_probe(h, args, B=None, horizon=None) -> dict:
             "yield_plus_grad_over_yield": (t_yield + t_grad) / t_yield, "psi0_recovery_error": rec}
 
 
+def gradcheck_probe(theta: float = 60.0, phi: float = 30.0, field_uT: float = 50.0, sample: int = 0,
+                    budget: float = 1e-12) -> dict:
+    """Adjoint gradient of one production trajectory over the whole 57.56 us horizon (46,052 output
+    intervals) against central differences (h = 1e-4 rad, 1e-3 uT) of the GPU Krylov yield, with a
+    tight error budget and the center run's step schedule replayed for the six perturbed runs (one
+    batched propagation). Returns the gradient, the differences and the relative errors."""
+    import math
+
+    import numpy as np
+    from rpm_amd import adjoint, batch, core
+    cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+    times = cfg.time_grid_us
+    base = core.build_hamiltonian(90.0, 0.05)
+    state = core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, sample)[None, :]
+    bp = batch.BatchedLanczosPropagator(base, state_error_budget=budget, initial_step_us=0.05)
+    build = lambda t, f_mT, p: core.build_hamiltonian(t, f_mT, phi_deg=p)  # noqa: E731
+    t0 = time.perf_counter()
+    out = adjoint.yield_and_gradient(base, build, bp, state, times, cfg.lifetime_us, [theta], [phi], [field_uT])
+    t_grad = time.perf_counter() - t0
+    center = bp.propagate(state, times, pairs=batch.pair_matrices(base, [build(theta, 1e-3 * field_uT, phi)]))
+    h_angle, h_field = 1e-4, 1e-3
+    shifts = [(math.degrees(h_angle), 0.0, 0.0), (0.0, math.degrees(h_angle), 0.0), (0.0, 0.0, h_field)]
+    params = []
+    for dt_, dp, df in shifts:
+        params += [(theta + dt_, phi + dp, field_uT + df), (theta - dt_, phi - dp, field_uT - df)]
+    pairs = batch.pair_matrices(base, [build(t, 1e-3 * f, p) for t, p, f in params])
+    res = bp.propagate(np.repeat(state, len(params), axis=0), times, pairs=pairs,
+                       schedules=[center[0].schedule] * len(params))
+    y = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
+    fd = np.array([(y[2 * k] - y[2 * k + 1]) / (2 * h) for k, h in enumerate((h_angle, h_angle, h_field))])
+    g = np.asarray(out.field_gradient[0], dtype=float)
+    scale = 1e-3 * np.abs(fd).max()
+    rel = [float(abs(g[k] - fd[k]) / max(abs(fd[k]), scale)) for k in range(3)]
+    return {"mode": "gradcheck", "theta_deg": theta, "phi_deg": phi, "field_uT": field_uT, "sample": sample,
+            "horizon_us": float(times[-1]), "intervals": int(times.size - 1), "budget": budget,
+            "steps": center[0].stats.accepted_steps, "seconds_yield_and_gradient": t_grad,
+            "parameters": ["theta (per rad)", "phi (per rad)", "|B| (per uT)"],
+            "adjoint": g.tolist(), "central_differences": fd.tolist(), "relative_error": rel,
+            "psi0_recovery_error": out.recovered_initial_error}
+
+
 def sobol_probe(args, cells: int = 16, q_points: int = 16) -> dict:
     """Batched Sobol reaction-time cells on the full system: tau = 1 us, Q points x 8 scrambles each,
     angles spread over 0..180 deg, one batched propagation."""
@@ -533,7 +575,7 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
-                             "trajectory", "archive_batch"),
+                             "trajectory", "archive_batch", "gradcheck"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -575,6 +617,9 @@ def main() -> int:
         return 0
     if args.mode == "archive_batch":
         print(json.dumps(archive_batch_probe()), flush=True)
+        return 0
+    if args.mode == "gradcheck":
+        print(json.dumps(gradcheck_probe()), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()      # the H·ψ measurement only
