@@ -18,6 +18,7 @@ trajectory of the reference's archived run, against its archived wall time).
     python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
     python bench.py --mode archive_batch   # 32 whole archived trajectories in one batch (about a minute)
     python bench.py --mode gradcheck       # whole-horizon adjoint gradient against central differences
+    python bench.py --mode pulse           # one whole-horizon ascent step of the 64-segment pulse
 """
 
 from __future__ import annotations
@@ -466,6 +467,45 @@ def gradcheck_probe(theta: float = 60.0, phi: float = 30.0, field_uT: float = 50
             "psi0_recovery_error": out.recovered_initial_error}
 
 
+def pulse_probe(vectors: int = 8, segments: int = 64, window_us: float = 2.0, max_move_uT: float = 1.0) -> dict:
+    """One ascent step of the field-pulse optimisation over the whole 57.56 us horizon: 50 uT static
+    along z plus a transverse (x) piecewise-constant pulse of `segments` amplitudes U(-20, 20) uT over
+    [0, window_us]; `vectors` random-phase states. Times the forward run and the adjoint sweep
+    (gradient with respect to every amplitude), then moves the amplitudes along the ensemble-mean
+    gradient (largest move `max_move_uT`) and runs the forward again."""
+    import numpy as np
+    import torch
+    from rpm_amd import adjoint, batch, core
+    cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
+    times = cfg.time_grid_us
+    base = core.build_hamiltonian(0.0, 0.05)
+    states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, i) for i in range(vectors)])
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
+    build = lambda t, f_mT, p: core.build_hamiltonian(t, f_mT, phi_deg=p)  # noqa: E731
+    prob = adjoint.FieldYield(base, build, bp, states, times, cfg.lifetime_us, pulse_window_us=(0.0, window_us),
+                              pulse_direction=(1.0, 0.0, 0.0))
+    amps = np.random.default_rng(11).uniform(-20.0, 20.0, size=segments)
+    fixed = ([0.0] * vectors, [0.0] * vectors, [50.0] * vectors)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y0 = prob.forward(*fixed, np.tile(amps, (vectors, 1)))
+    torch.cuda.synchronize()
+    t_forward = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, pgrad, rec = prob.backward()
+    torch.cuda.synchronize()
+    t_sweep = time.perf_counter() - t0
+    g = pgrad.mean(axis=0)
+    step = max_move_uT / float(np.abs(g).max())
+    y1 = prob.forward(*fixed, np.tile(amps + step * g, (vectors, 1)))
+    return {"mode": "pulse", "vectors": vectors, "segments": segments, "window_us": window_us,
+            "horizon_us": float(times[-1]), "seconds_forward": t_forward, "seconds_sweep": t_sweep,
+            "seconds_per_vector_step": (t_forward + t_sweep) / vectors,
+            "yield_plus_grad_over_yield": (t_forward + t_sweep) / t_forward,
+            "mean_yield": float(y0.mean()), "mean_yield_after_step": float(y1.mean()),
+            "gradient_max_abs_per_uT": float(np.abs(g).max()), "psi0_recovery_error": rec}
+
+
 def sobol_probe(args, cells: int = 16, q_points: int = 16) -> dict:
     """Batched Sobol reaction-time cells on the full system: tau = 1 us, Q points x 8 scrambles each,
     angles spread over 0..180 deg, one batched propagation."""
@@ -575,7 +615,7 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
-                             "trajectory", "archive_batch", "gradcheck"),
+                             "trajectory", "archive_batch", "gradcheck", "pulse"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -620,6 +660,9 @@ def main() -> int:
         return 0
     if args.mode == "gradcheck":
         print(json.dumps(gradcheck_probe()), flush=True)
+        return 0
+    if args.mode == "pulse":
+        print(json.dumps(pulse_probe()), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()      # the H·ψ measurement only
