@@ -496,7 +496,8 @@ def pulse_probe(vectors: int = 8, segments: int = 64, window_us: float = 2.0, ma
     torch.cuda.synchronize()
     t_sweep = time.perf_counter() - t0
     g = pgrad.mean(axis=0)
-    step = max_move_uT / float(np.abs(g).max())
+    gmax = float(np.abs(g).max())
+    step = max_move_uT / gmax if gmax > 0.0 else 0.0
     y1 = prob.forward(*fixed, np.tile(amps + step * g, (vectors, 1)))
     return {"mode": "pulse", "vectors": vectors, "segments": segments, "window_us": window_us,
             "horizon_us": float(times[-1]), "seconds_forward": t_forward, "seconds_sweep": t_sweep,
