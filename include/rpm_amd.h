@@ -182,7 +182,7 @@ int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k, int batch
                            void* d_workspace, size_t workspace_bytes, void* stream);
 
 /* Batched rpm_krylov_combine: d_out + b*out_ld = scale[b] * sum_{j<k_b} basis_b[j] c_b[j],
- * basis_b = d_basis + b*m*n, c_b = d_c + b*c_stride, k_b = d_k[b] (<= 0 skips b). */
+ * basis_b = d_basis + b*m*n, c_b = d_c + b*c_stride, k_b = min(d_k[b], m, c_stride) (<= 0 skips b). */
 int rpm_krylov_combine_batch(const void* d_basis, int m, const int* d_k, int batch, long long n,
                              const void* d_c, int c_stride, const double* d_scale, void* d_out,
                              long long out_ld, void* stream);
@@ -213,8 +213,9 @@ int rpm_axis_apply(int naxes, const int* dims, int axis, const void* d_U, const 
  * nterms[c] = K <= kmax, coefficients coef[c][0..K] (host, interleaved complex) of
  * exp(-i H (t_b - t_a)) = sum_n c_n T_n(X), X = (H - shift)/half_width, and the (kmax+1)^2 complex
  * matrix d_G[c] (device) with G[m][j] = sum_{kicks t in (t_a, t_b]} 2 w(t) conj(c_m(t - t_a)) c_j(t - t_a)),
- * a weight multiplier mult[k] (W = mult[k] w), electron-matrix segment seg[k] (d_pairs: nseg x batch
- * x 16 complex, nullable) and accumulator slot slot[k].  Running backward from d_psi = psi(t_end)
+ * a weight multiplier mult[k] (W = mult[k] w), electron-matrix segment seg[k] < nseg (d_pairs: nseg x batch
+ * x 16 complex, nullable; seg and nseg are read only with d_pairs) and accumulator slot slot[k] < nslot
+ * (d_acc: nslot x batch x 16 complex).  Out-of-range classes, slots or segments give RPM_E_INVALID.  Running backward from d_psi = psi(t_end)
  * (overwritten with psi(0)), each step carries psi back by Clenshaw's recurrence, forms
  * u_j = T_j(X) psi(t_a), and runs Clenshaw over the costate with the vector seeds
  * z_m = conj(c_m) g + mult P sum_j G[m][j] u_j, accumulating
@@ -226,8 +227,8 @@ size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch, int kmax);
 int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro, double shift,
                       double half_width, int ncls, const int* nterms, int kmax, const double* coef,
                       const void* d_G, const int* cls, const double* mult, const void* d_pairs,
-                      const int* seg, void* d_acc, const int* slot, void* d_workspace, size_t workspace_bytes,
-                      void* stream);
+                      const int* seg, int nseg, void* d_acc, const int* slot, int nslot, void* d_workspace,
+                      size_t workspace_bytes, void* stream);
 
 /*
  * Factorized spectral yields (R/rpm_quantum_needle_reproduction.py:145-172, transition_lines and

@@ -295,13 +295,15 @@ def run_trajectory(hamiltonian, initial_state, config: SweepConfig, propagator=N
 
 
 def run_segments(hamiltonian, initial_state, times_us, boundaries, make_prop, state_error_budget, axis=None,
-                 unitary=None):
+                 unitary=None, *, hamiltonians=None, schedules=None):
     """Segment loop of R/rpm_nuclear_spin_rotation_control.py:247-342 (test oracle).
 
     ``boundaries``: pulse times (the final time is appended); ``make_prop(h)``
     builds a fresh propagator per segment; the local unitary acts on ``axis``
-    after every boundary but the last.  Returns (populations, bounds,
-    accumulated state error, final state, schedules).
+    after every boundary but the last.  Extensions for piecewise-constant
+    fields: ``hamiltonians`` gives one Hamiltonian per segment (len(boundaries)
+    + 1), ``schedules`` one recorded schedule per segment to replay.  Returns
+    (populations, bounds, accumulated state error, final state, schedules).
     """
     times = np.asarray(times_us, float)
     dims = tuple(hamiltonian.dimensions)
@@ -312,8 +314,11 @@ def run_segments(hamiltonian, initial_state, times_us, boundaries, make_prop, st
     pops[0] = singlet_population(state, dims)
     bounds[0] = 0.0
     cursor, current, acc = 1, 0.0, 0.0
-    schedules = []
-    for boundary in np.append(np.asarray(boundaries, float), times[-1]):
+    schedules_out = []
+    ends = np.append(np.asarray(boundaries, float), times[-1])
+    if hamiltonians is not None and len(hamiltonians) != ends.size:
+        raise ValueError("hamiltonians must give one Hamiltonian per segment")
+    for index, boundary in enumerate(ends):
         boundary = float(boundary)
         stop = int(np.searchsorted(times, boundary + 1e-12, side="right"))
         requested = times[cursor:stop] - current
@@ -321,17 +326,18 @@ def run_segments(hamiltonian, initial_state, times_us, boundaries, make_prop, st
         local = requested if (requested.size and requested[-1] >= duration - 1e-12) else np.append(requested,
                                                                                                      duration)
         if duration > 0.0:
-            prop = make_prop(hamiltonian)
+            prop = make_prop(hamiltonian if hamiltonians is None else hamiltonians[index])
             prop.state_error_budget = state_error_budget * duration / times[-1]
-            res = prop.propagate(state, local, obs)
+            res = prop.propagate(state, local, obs,
+                                 schedule=None if schedules is None else schedules[len(schedules_out)])
             count = requested.size
             pops[cursor:stop] = res.observable_values[:count]
             e = acc + res.state_error_estimates[:count]
             bounds[cursor:stop] = 2.0 * e + e ** 2
             state = res.final_state
             acc += res.stats.accumulated_state_error
-            schedules.append(res.schedule)
+            schedules_out.append(res.schedule)
         cursor, current = stop, boundary
         if unitary is not None and boundary < times[-1] - 1e-12:
             state = local_unitary_apply(state, dims, axis, unitary)
-    return pops, bounds, acc, state, schedules
+    return pops, bounds, acc, state, schedules_out

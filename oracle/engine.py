@@ -460,8 +460,15 @@ class AdaptiveLanczosPropagator:
         return c
 
     def propagate(self, initial_state, times_us, observable, *, store_states=False,
-                  progress_callback=None) -> PropagationResult:
-        """R/rpm_statevector_engine.py:717-933 (rule set restated one for one)."""
+                  progress_callback=None, schedule=None) -> PropagationResult:
+        """R/rpm_statevector_engine.py:717-933 (rule set restated one for one).
+
+        ``schedule`` (test extension, no reference counterpart): a recorded
+        schedule as returned in ``PropagationResult.schedule``.  Its trial steps,
+        accept/reject decisions and selected dimensions are replayed instead of
+        being decided from this run's error estimates, so that runs at nearby
+        parameters share one step sequence (central finite differences).
+        """
         times = np.asarray(tuple(float(t) for t in times_us), dtype=float)
         if times.ndim != 1 or times.size == 0:
             raise ValueError("At least one observation time is required.")
@@ -481,7 +488,8 @@ class AdaptiveLanczosPropagator:
         cursor, now, acc_err = 0, 0.0, 0.0
         accepted = rejected = max_dim = 0
         used = []
-        schedule = []
+        replay = None if schedule is None else list(schedule)
+        record = []
         max_norm_err = abs(float(np.linalg.norm(state)) - 1.0)
         mv0 = self.hamiltonian.matvec_count
         if times[0] == 0.0:
@@ -496,6 +504,12 @@ class AdaptiveLanczosPropagator:
 
         while cursor < times.size:
             trial = min(step, t_final - now)
+            rec = None
+            if replay is not None:
+                if len(record) >= len(replay):
+                    raise RuntimeError("Replayed schedule ended before the final time.")
+                rec = replay[len(record)]
+                trial = float(rec[0])
             if trial < self.minimum_step_us:
                 raise RuntimeError("Lanczos step fell below the configured minimum.")
             snorm = float(np.linalg.norm(state))
@@ -503,6 +517,12 @@ class AdaptiveLanczosPropagator:
             basis, d, e = self._basis(state, target)
             sizes = [d.size]
             budget = self.state_error_budget * trial / max(t_final, trial)
+
+            def ok(dim, err, rec=rec, budget=budget):
+                if rec is None:
+                    return err <= budget
+                return bool(rec[1]) and int(dim) == int(rec[2])
+
             selected, c_end, err = None, None, math.inf
             exact = d.size < target
             if exact:
@@ -510,7 +530,7 @@ class AdaptiveLanczosPropagator:
                 c_end = projected_coefficients(d, e, trial)
             elif not prefer_max:
                 c_end, err = self._candidate(d, e, self.minimum_dimension, trial, snorm)
-                if err <= budget:
+                if ok(self.minimum_dimension, err):
                     selected = self.minimum_dimension
                 elif self.maximum_dimension > self.minimum_dimension:
                     basis, d, e = self._basis(state, self.maximum_dimension)
@@ -523,22 +543,24 @@ class AdaptiveLanczosPropagator:
                     else:
                         for dim in self._ladder(self.minimum_dimension + self.dimension_increment):
                             c_end, err = self._candidate(d, e, dim, trial, snorm)
-                            if err <= budget:
+                            if ok(dim, err):
                                 selected = dim
                                 break
             else:
                 for dim in self._ladder(self.minimum_dimension):
                     c_end, err = self._candidate(d, e, dim, trial, snorm)
-                    if err <= budget:
+                    if ok(dim, err):
                         selected = dim
                         break
-            if selected is None or err > budget:
+            if exact and rec is not None and not rec[1]:
+                selected = None
+            if selected is None or (rec is None and err > budget):
                 rejected += 1
-                schedule.append((trial, False, 0, tuple(sizes)))
+                record.append((trial, False, 0, tuple(sizes)))
                 step = 0.5 * trial
                 continue
 
-            schedule.append((trial, True, int(selected), tuple(sizes)))
+            record.append((trial, True, int(selected), tuple(sizes)))
             max_dim = max(max_dim, selected)
             if selected > self.minimum_dimension:
                 prefer_max = True
@@ -582,4 +604,4 @@ class AdaptiveLanczosPropagator:
         stats = PropagationStats(accepted, rejected, self.hamiltonian.matvec_count - mv0, max_dim,
                                  min(used) if used else 0.0, max(used) if used else 0.0,
                                  acc_err, max_norm_err)
-        return PropagationResult(times, values, bounds, errors, stored, state, stats, tuple(schedule))
+        return PropagationResult(times, values, bounds, errors, stored, state, stats, tuple(record))

@@ -80,7 +80,8 @@ SIGNATURES = {
     "rpm_adjoint_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
     "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,
-                                         _c_double_p, _vp, _c_int_p, _vp, _c_int_p, _vp, _sz, _vp]),
+                                         _c_double_p, _vp, _c_int_p, ctypes.c_int, _vp, _c_int_p, ctypes.c_int,
+                                         _vp, _sz, _vp]),
 }
 
 _lib = None

@@ -259,8 +259,9 @@ def adjoint_densities(hamiltonian, final_states, times_us, weights, *, pairs=Non
         nterms.ctypes.data_as(ip), int(kmax), coef_ri.ctypes.data_as(dp), ctypes.c_void_p(nat.ptr(dG)),
         cls_c.ctypes.data_as(ip), mult_c.ctypes.data_as(dp),
         ctypes.c_void_p(nat.ptr(d_pairs) if d_pairs is not None else None),
-        step_seg.ctypes.data_as(ip) if d_pairs is not None else None, ctypes.c_void_p(nat.ptr(acc)),
-        step_slot.ctypes.data_as(ip), ctypes.c_void_p(nat.ptr(ws)), ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
+        step_seg.ctypes.data_as(ip) if d_pairs is not None else None, int(p.shape[0]) if p is not None else 0,
+        ctypes.c_void_p(nat.ptr(acc)), step_slot.ctypes.data_as(ip), int(nslot), ctypes.c_void_p(nat.ptr(ws)),
+        ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
     dens = acc.cpu().numpy().reshape(nslot, B, 4, 4)
     return AdjointResult(dens, psi, tuple(int(c.K) for c in classes), (lo, hi), len(steps))
 
@@ -372,15 +373,24 @@ class FieldYield:
         yields = np.array([integrate_yield(times, pops[b], self.tau) for b in range(B)])
         dH = np.stack([field_parameter_derivatives(float(t), float(f), float(p))
                        for t, p, f in zip(thetas_deg, phis_deg, fields_uT)])
+        # a new record per call: earlier records stay valid for their own backward()
         self._saved = dict(final=final.clone(), grid=grid, w=w, seg_pairs=seg_pairs, seg_idx=seg_idx,
                            nseg=seg_pairs.shape[0], pulse_slots=pulse_slots, dH=dH, pops=pops)
         return yields
 
-    def backward(self):
-        """(field gradient (B, 3) per rad, rad, uT; pulse gradient (B, S) per uT or None; psi_0 recovery error)."""
-        if self._saved is None:
+    def saved_state(self):
+        """The record the last forward() kept for backward(); pass it to backward() to differentiate that call."""
+        return self._saved
+
+    def backward(self, saved=None):
+        """(field gradient (B, 3) per rad, rad, uT; pulse gradient (B, S) per uT or None; psi_0 recovery error).
+
+        ``saved``: the record of the forward call to differentiate (``saved_state()``
+        right after that call); by default the most recent forward call's.
+        """
+        sv = self._saved if saved is None else saved
+        if sv is None:
             raise RuntimeError("backward() needs a forward() first")
-        sv = self._saved
         adj = adjoint_densities(self.base, sv["final"], sv["grid"], sv["w"], pairs=sv["seg_pairs"],
                                 segment_of_interval=sv["seg_idx"], slot_of_interval=sv["seg_idx"], nslot=sv["nseg"],
                                 decay_time=self.tau)
@@ -463,12 +473,14 @@ def _autograd_function():
             amps = None if pulse_amplitudes is None else pulse_amplitudes.detach().cpu().double().numpy()
             y = problem.forward(np.degrees(p[:, 0]), np.degrees(p[:, 1]), p[:, 2], amps)
             ctx.problem = problem
+            # this call's own record: a later forward on the same problem must not change its gradient
+            ctx.saved = problem.saved_state()
             ctx.has_pulse = pulse_amplitudes is not None
             return torch.from_numpy(y).to(field_params.device)
 
         @staticmethod
         def backward(ctx, grad_out):
-            fgrad, pgrad, _ = ctx.problem.backward()
+            fgrad, pgrad, _ = ctx.problem.backward(ctx.saved)
             go = grad_out.detach().cpu().double().numpy()[:, None]
             gf = torch.from_numpy(go * fgrad).to(grad_out.device)
             gp = torch.from_numpy(go * pgrad).to(grad_out.device) if ctx.has_pulse else None

@@ -169,13 +169,14 @@ class BatchedLanczosPropagator:
 
         while cursor < nt:
             trial = min(step, t_final - now)
-            if trial < self.minimum_step_us:
-                raise RuntimeError("Lanczos step fell below the configured minimum.")
             if replay is not None:
                 if len(record) >= len(replay):
                     raise RuntimeError("Replayed schedule ended before the final time.")
                 rec = replay[len(record)]
-
+                trial = float(rec[0])
+            if trial < self.minimum_step_us:
+                raise RuntimeError("Lanczos step fell below the configured minimum.")
+            if replay is not None:
                 def ok(dim, e, rec=rec):
                     return bool(rec[1]) and int(dim) == int(rec[2])
             else:

@@ -278,16 +278,20 @@ extern "C" size_t rpm_adjoint_workspace_bytes(const rpm_operator* op, int batch,
 extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi, int nmacro, double shift,
                                  double half_width, int ncls, const int* nterms, int kmax, const double* coef,
                                  const void* d_G, const int* cls, const double* mult, const void* d_pairs,
-                                 const int* seg, void* d_acc, const int* slot, void* d_ws, size_t ws_bytes,
-                                 void* stream) {
+                                 const int* seg, int nseg, void* d_acc, const int* slot, int nslot, void* d_ws,
+                                 size_t ws_bytes, void* stream) {
   if (!op || op->kind != 1 || !d_psi || nmacro < 1 || !nterms || !coef || !d_G || !cls || !mult || !d_acc ||
-      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || kmax > 127 || !(half_width > 0.0))
+      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || kmax > 127 || !(half_width > 0.0) ||
+      nslot < 1)
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: bad arguments");
-  if (d_pairs && !seg) return fail(RPM_E_INVALID, "rpm_adjoint_sweep: per-step segments needed with pairs");
+  if (d_pairs && (!seg || nseg < 1))
+    return fail(RPM_E_INVALID, "rpm_adjoint_sweep: per-step segments and their count needed with pairs");
   for (int c = 0; c < ncls; ++c)
     if (nterms[c] < 2 || nterms[c] > kmax) return fail(RPM_E_INVALID, "Chebyshev length %d outside 2..%d", nterms[c], kmax);
+  // every index the kernels use to address d_G, d_acc and d_pairs is checked here, on the host
   for (int k = 0; k < nmacro; ++k)
-    if (cls[k] < 0 || cls[k] >= ncls || slot[k] < 0 || (seg && seg[k] < 0))
+    if (cls[k] < 0 || cls[k] >= ncls || slot[k] < 0 || slot[k] >= nslot ||
+        (d_pairs && (seg[k] < 0 || seg[k] >= nseg)))
       return fail(RPM_E_INVALID, "rpm_adjoint_sweep: step %d has a bad class, slot or segment", k);
   if (ws_bytes < rpm_adjoint_workspace_bytes(op, batch, kmax))
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: workspace too small");

@@ -19,6 +19,7 @@
 // kernel reads it and returns at once when the trajectory is idle at the
 // current Lanczos index (breakdown, or a smaller target), so the host never
 // waits inside the basis loop.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 
@@ -456,7 +457,7 @@ __global__ void __launch_bounds__(kThreads) k_combine(const c128* __restrict__ V
                                                       const double* __restrict__ scale, c128* __restrict__ out,
                                                       long long ostride) {
   const int b = blockIdx.y;
-  const int kk = kb ? kb[b] : k;
+  const int kk = kb ? min(kb[b], k) : k;   // with per-trajectory counts, k is their upper limit
   if (kk <= 0) return;
   const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
@@ -786,12 +787,14 @@ extern "C" int rpm_krylov_combine(const void* d_basis, int k, long long n, const
 extern "C" int rpm_krylov_combine_batch(const void* d_basis, int m, const int* d_k, int batch, long long n,
                                         const void* d_c, int c_stride, const double* d_scale, void* d_out,
                                         long long out_ld, void* stream) {
-  if (!d_basis || !d_k || !d_c || !d_out || batch < 1 || batch > 65535 || n < 1 || m < 1 || out_ld < n)
+  if (!d_basis || !d_k || !d_c || !d_out || batch < 1 || batch > 65535 || n < 1 || m < 1 || c_stride < 1 ||
+      out_ld < n)
     return fail(RPM_E_INVALID, "rpm_krylov_combine_batch: bad arguments");
+  // the device counts d_k cannot be checked here: the kernel clamps them to the basis and coefficient rows
   hipLaunchKernelGGL(k_combine, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0,
                      static_cast<hipStream_t>(stream), static_cast<const c128*>(d_basis),
-                     static_cast<long long>(m) * n, 0, d_k, n, static_cast<const c128*>(d_c), c_stride, d_scale,
-                     static_cast<c128*>(d_out), out_ld);
+                     static_cast<long long>(m) * n, std::min(m, c_stride), d_k, n, static_cast<const c128*>(d_c),
+                     c_stride, d_scale, static_cast<c128*>(d_out), out_ld);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

@@ -677,8 +677,8 @@ class AdaptiveLanczosPropagator:
         ``schedule`` (extension): a recorded step schedule -- one
         ``(trial_step_us, accepted, selected_dimension, basis_sizes)`` per
         attempt, as returned in ``PropagationResult.schedule`` -- whose
-        accept/reject decisions and selected dimensions are replayed instead of
-        being re-decided from this run's error estimates.
+        trial steps, accept/reject decisions and selected dimensions are replayed
+        instead of being re-decided from this run's error estimates.
         """
         torch = nat.require_gpu()
         times = np.asarray(tuple(float(t) for t in times_us), dtype=float)
@@ -730,13 +730,15 @@ class AdaptiveLanczosPropagator:
 
         while cursor < nt:
             trial = min(step, t_final - now)
-            if trial < self.minimum_step_us:
-                raise RuntimeError("Lanczos step fell below the configured minimum.")
+            rec = None
             if replay is not None:
                 if len(record) >= len(replay):
                     raise RuntimeError("Replayed schedule ended before the final time.")
                 rec = replay[len(record)]
-
+                trial = float(rec[0])
+            if trial < self.minimum_step_us:
+                raise RuntimeError("Lanczos step fell below the configured minimum.")
+            if rec is not None:
                 def passes(dim, err, rec=rec):
                     return bool(rec[1]) and int(dim) == int(rec[2])
             else:

@@ -133,12 +133,14 @@ def _check_segments(times, segments):
 
 
 def propagate_segments(hamiltonian, initial_state, times_us, segments: Sequence[Segment], make_propagator: Callable,
-                       state_error_budget: float, *, hamiltonians=None) -> SegmentedResult:
+                       state_error_budget: float, *, hamiltonians=None, schedules=None) -> SegmentedResult:
     """Single trajectory through the reference's segment loop.
 
     ``make_propagator(h)`` builds a fresh ``AdaptiveLanczosPropagator`` per
     segment (R/poster_reproduction/core.py:232-247); ``hamiltonians``
-    optionally gives one Hamiltonian per segment (piecewise-constant fields).
+    optionally gives one Hamiltonian per segment (piecewise-constant fields);
+    ``schedules`` optionally gives one recorded schedule per propagated segment
+    to replay (as ``SegmentedResult.schedules`` holds them).
     """
     times = np.asarray(times_us, dtype=float)
     _check_segments(times, segments)
@@ -153,7 +155,7 @@ def propagate_segments(hamiltonian, initial_state, times_us, segments: Sequence[
     bounds[0] = 0.0
     cursor, current, acc = 1, 0.0, 0.0
     accepted = rejected = matvecs = 0
-    schedules = []
+    schedule_log = []
     for i, seg in enumerate(segments):
         boundary = float(seg.end_us)
         h = hamiltonian if hamiltonians is None else hamiltonians[i]
@@ -161,7 +163,8 @@ def propagate_segments(hamiltonian, initial_state, times_us, segments: Sequence[
         if duration > 0.0:
             prop = make_propagator(h)
             prop.state_error_budget = state_error_budget * duration / times[-1]
-            res = prop.propagate(state, local, obs)
+            replay = None if schedules is None else schedules[len(schedule_log)][0]
+            res = prop.propagate(state, local, obs, schedule=replay)
             count = requested.size
             pops[cursor:stop] = res.observable_values[:count]
             e = acc + res.state_error_estimates[:count]
@@ -171,21 +174,23 @@ def propagate_segments(hamiltonian, initial_state, times_us, segments: Sequence[
             rejected += res.stats.rejected_steps
             matvecs += res.stats.matvecs
             acc += res.stats.accumulated_state_error
-            schedules.append([res.schedule])
+            schedule_log.append([res.schedule])
         cursor, current = stop, boundary
         if seg.unitary is not None and boundary < times[-1] - 1e-12:
             axis, U = seg.unitary
             state = local_unitary_apply(state, dims, axis, U)
     return SegmentedResult(times, pops[None], bounds[None], np.array([acc]), np.array([accepted]),
-                           np.array([rejected]), np.array([matvecs]), state.reshape(1, -1), schedules)
+                           np.array([rejected]), np.array([matvecs]), state.reshape(1, -1), schedule_log)
 
 
 def propagate_segments_batched(propagator, initial_states, times_us, segments: Sequence[Segment],
-                               state_error_budget: float) -> SegmentedResult:
+                               state_error_budget: float, *, schedules=None) -> SegmentedResult:
     """A batch of trajectories through the same segment loop with ``BatchedLanczosPropagator``.
 
     Each segment may give per-trajectory electron matrices (field pulses);
     ``propagator.state_error_budget`` is set per segment as in the reference.
+    ``schedules``: optional recorded schedules to replay, one list per
+    propagated segment with one entry per trajectory (entries may be None).
     """
     torch = nat.require_gpu()
     times = np.asarray(times_us, dtype=float)
@@ -209,7 +214,7 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
     accepted = np.zeros(B, dtype=int)
     rejected = np.zeros(B, dtype=int)
     matvecs = np.zeros(B, dtype=int)
-    schedules = []
+    schedule_log = []
     for seg in segments:
         boundary = float(seg.end_us)
         stop, requested, duration, local = _local_times(times, cursor, boundary, current)
@@ -220,7 +225,8 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
                 if pairs.shape == (4, 4):
                     pairs = np.broadcast_to(pairs, (B, 4, 4))
             propagator.state_error_budget = state_error_budget * duration / times[-1]
-            res = propagator.propagate(states, local, pairs=pairs)
+            replay = None if schedules is None else schedules[len(schedule_log)]
+            res = propagator.propagate(states, local, pairs=pairs, schedules=replay)
             count = requested.size
             for b, r in enumerate(res):
                 pops[b, cursor:stop] = r.observable_values[:count]
@@ -231,12 +237,12 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
                 matvecs[b] += r.stats.matvecs
                 acc[b] += r.stats.accumulated_state_error
             states = propagator.final_states
-            schedules.append([r.schedule for r in res])
+            schedule_log.append([r.schedule for r in res])
         cursor, current = stop, boundary
         if seg.unitary is not None and boundary < times[-1] - 1e-12:
             axis, U = seg.unitary
             states = local_unitary_apply(states, dims, axis, U)
-    return SegmentedResult(times, pops, bounds, acc, accepted, rejected, matvecs, states, schedules)
+    return SegmentedResult(times, pops, bounds, acc, accepted, rejected, matvecs, states, schedule_log)
 
 
 def field_pulse_segments(base_pair: np.ndarray, direction, amplitudes_uT, t_start_us: float, t_end_us: float,

@@ -67,3 +67,45 @@ def test_batched_propagator_edges_and_ragged_times():
         assert (g.stats.accepted_steps, g.stats.matvecs) == (one.stats.accepted_steps, one.stats.matvecs)
     only0 = bp.propagate(states, [0.0])
     assert all(r.stats.matvecs == 0 for r in only0)
+
+
+def test_adjoint_sweep_rejects_out_of_range_slots_and_segments():
+    """rpm_adjoint_sweep checks every step's slot and segment against nslot / nseg on the host, before any
+    launch, so a bad index gives RPM_E_INVALID instead of a write past the accumulators."""
+    import ctypes
+
+    from rpm_amd import _native as nat
+    print("case: adjoint sweep index checks", flush=True)
+    h, s0, _ = _small()
+    lib = nat.load()
+    op = h._device_op()
+    B, kmax = 1, 4
+    psi = torch.from_numpy(s0.reshape(1, -1)).cuda()
+    ws = nat.workspace(lib.rpm_adjoint_workspace_bytes(ctypes.c_void_p(op.handle), B, kmax))
+    acc = torch.zeros((2, B, 16), dtype=torch.complex128, device=psi.device)
+    pairs = torch.zeros((2, B, 16), dtype=torch.complex128, device=psi.device)
+    G = torch.zeros((1, kmax + 1, kmax + 1), dtype=torch.complex128, device=psi.device)
+    nterms = np.array([kmax], dtype=np.int32)
+    coef = np.zeros(2 * (kmax + 1))
+    cls = np.zeros(1, dtype=np.int32)
+    mult = np.ones(1)
+    ip = ctypes.POINTER(ctypes.c_int)
+    dp = ctypes.POINTER(ctypes.c_double)
+
+    def call(slot, nslot, seg, nseg):
+        sl = np.array([slot], dtype=np.int32)
+        sg = np.array([seg], dtype=np.int32)
+        return lib.rpm_adjoint_sweep(
+            ctypes.c_void_p(op.handle), B, ctypes.c_void_p(nat.ptr(psi)), 1, 0.0, 1.0, 1, nterms.ctypes.data_as(ip),
+            kmax, coef.ctypes.data_as(dp), ctypes.c_void_p(nat.ptr(G)), cls.ctypes.data_as(ip),
+            mult.ctypes.data_as(dp), ctypes.c_void_p(nat.ptr(pairs)), sg.ctypes.data_as(ip), nseg,
+            ctypes.c_void_p(nat.ptr(acc)), sl.ctypes.data_as(ip), nslot, ctypes.c_void_p(nat.ptr(ws)), ws.numel(),
+            ctypes.c_void_p(nat.stream_ptr()))
+
+    assert call(2, 2, 0, 2) == nat.RPM_E_INVALID      # slot == nslot
+    assert call(-1, 2, 0, 2) == nat.RPM_E_INVALID
+    assert call(0, 2, 2, 2) == nat.RPM_E_INVALID      # segment == nseg
+    assert call(0, 2, 0, 0) == nat.RPM_E_INVALID      # pairs without a segment count
+    assert call(0, 0, 0, 2) == nat.RPM_E_INVALID      # no accumulator slot
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(acc).item() == 0       # nothing ran

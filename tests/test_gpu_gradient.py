@@ -70,7 +70,9 @@ def test_field_gradient_small_system_matches_dense_finite_differences():
         assert max(errs) <= TOL
 
 
-def test_pulse_gradient_small_system_matches_dense_finite_differences():
+@pytest.mark.parametrize("window", [(0.0, 0.15), (0.05, 0.2)])
+def test_pulse_gradient_small_system_matches_dense_finite_differences(window):
+    print("case: pulse window", window, flush=True)
     sys_ = workloads.small_system(0)
     base = sys_.hamiltonian()
     times = np.arange(241) * 0.00125
@@ -81,13 +83,15 @@ def test_pulse_gradient_small_system_matches_dense_finite_differences():
     state = core.random_phase_initial_state(base.dimensions, 0, 3)[None]
     out = adjoint.yield_and_gradient(
         base, lambda t, fm, p: sys_.hamiltonian(theta_deg=t, field_mT=fm, phi_deg=p), _small_prop(base),
-        state, times, tau, [th], [ph], [f], pulse_amplitudes_uT=amps, pulse_window_us=(0.0, 0.15))
+        state, times, tau, [th], [ph], [f], pulse_amplitudes_uT=amps, pulse_window_us=window)
     h0 = _dense_h(sys_, th, f, ph)
     mx = np.kron(adjoint.electron_derivative_operators()[0], np.eye(h0.shape[0] // 4))
 
     def dense_yield(a):
-        edges = np.linspace(0.0, 0.15, a.size + 1)
-        pieces = [(edges[s + 1], h0 + 1e-3 * a[s] * mx) for s in range(a.size)] + [(times[-1], h0)]
+        # a window starting after 0 begins with a static-field piece
+        edges = np.linspace(window[0], window[1], a.size + 1)
+        lead = [(window[0], h0)] if window[0] > 0.0 else []
+        pieces = lead + [(edges[s + 1], h0 + 1e-3 * a[s] * mx) for s in range(a.size)] + [(times[-1], h0)]
         return odense.dense_grid_yield(pieces, state[0], times, tau)[0]
 
     assert abs(out.yields[0] - dense_yield(amps)) < 1e-9
@@ -155,3 +159,27 @@ def test_autograd_function_matches_direct_gradient():
     w = np.array([1.0, -2.0])[:, None]
     np.testing.assert_allclose(params.grad.numpy(), w * direct.field_gradient, rtol=1e-12, atol=1e-18)
     np.testing.assert_allclose(amps.grad.numpy(), w * direct.pulse_gradient, rtol=1e-12, atol=1e-18)
+
+
+def test_autograd_two_forwards_then_one_backward():
+    """A forward whose backward runs after a second forward keeps its own gradient."""
+    sys_ = workloads.small_system(0)
+    base = sys_.hamiltonian()
+    times = np.arange(81) * 0.00125
+    states = np.stack([core.random_phase_initial_state(base.dimensions, 0, i) for i in range(2)])
+    build = lambda t, f, p: sys_.hamiltonian(theta_deg=t, field_mT=f, phi_deg=p)  # noqa: E731
+    prob = adjoint.FieldYield(base, build, _small_prop(base), states, times, 1.0)
+    p1 = torch.tensor([[0.4, 0.1, 50.0], [1.2, -0.3, 40.0]], dtype=torch.float64, requires_grad=True)
+    p2 = torch.tensor([[2.0, 0.7, 30.0], [0.2, 1.1, 60.0]], dtype=torch.float64, requires_grad=True)
+    y1 = adjoint.singlet_yield(prob, p1)
+    y2 = adjoint.singlet_yield(prob, p2)
+    y1.sum().backward()
+    assert p2.grad is None
+    direct1 = adjoint.yield_and_gradient(base, build, _small_prop(base), states, times, 1.0,
+                                         np.degrees([0.4, 1.2]), np.degrees([0.1, -0.3]), [50.0, 40.0])
+    np.testing.assert_allclose(p1.grad.numpy(), direct1.field_gradient, rtol=1e-12, atol=1e-18)
+    y2.sum().backward()
+    direct2 = adjoint.yield_and_gradient(base, build, _small_prop(base), states, times, 1.0,
+                                         np.degrees([2.0, 0.2]), np.degrees([0.7, 1.1]), [30.0, 60.0])
+    np.testing.assert_allclose(p2.grad.numpy(), direct2.field_gradient, rtol=1e-12, atol=1e-18)
+    assert np.abs(direct1.field_gradient - direct2.field_gradient).max() > 1e-6
