@@ -143,6 +143,19 @@ int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, v
                       void* stream);
 
 /*
+ * rpm_lanczos_basis without a workspace argument and without any device allocation: no counterpart in
+ * the reference.  For callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments and results as rpm_lanczos_basis (bitwise: the same
+ * kernels); the workspace is a 16 MiB array in device memory that the library holds per device for the
+ * life of the process, enough for N = 221,184 at target 64.  A basis whose workspace would exceed it
+ * gives RPM_E_INVALID.  Every call on a device shares that device's array, so calls that overlap on one
+ * device (other host threads or other streams) race and must be serialised by the caller.
+ */
+int rpm_lanczos_basis_noalloc(const rpm_operator* op, const void* d_state, int target, void* d_basis,
+                              double* d_alpha, double* d_beta, int* d_info, double* d_norm,
+                              double breakdown_tolerance, void* stream);
+
+/*
  * Batched basis (as rpm_lanczos_basis): trajectory b < batch starts from
  * d_states + b*ld (ld == N) and
  * builds min(d_targets[b], max_target) vectors into d_basis + b*m*N (vector j

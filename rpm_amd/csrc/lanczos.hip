@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "rpm_common.hpp"
 
@@ -757,6 +758,65 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
   return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
                      static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
                      d_norm, tol, d_ws, ws_bytes, s);
+}
+
+// ---------------------------------------------------------------- allocation-free basis
+
+// Scratch of rpm_lanczos_basis_noalloc: the single-trajectory basis workspace (two work vectors, the
+// pass's column partials, control blocks and counters; basis_layout) lives in this array instead of
+// caller or library allocations.  16 MiB holds it for the production system (N = 221,184, m = 64:
+// about 8 MB) and for any N up to about 450,000 at m = 64.  Each device keeps its own copy for the
+// life of the process.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// Device address of the current device's copy of `scratch`, looked up once per device (with that
+// device current) by hipGetSymbolAddress.  In host code the array's own name is not a device address.
+int scratch_address(void** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(RPM_E_HIP, "rpm_lanczos_basis_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxScratchDevices)
+    return fail(RPM_E_UNSUPPORTED, "rpm_lanczos_basis_noalloc: device %d above %d", dev, kMaxScratchDevices - 1);
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(RPM_E_HIP, "rpm_lanczos_basis_noalloc: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return RPM_OK;
+}
+}  // namespace
+
+extern "C" int rpm_lanczos_basis_noalloc(const rpm_operator* op, const void* d_state, int target, void* d_basis,
+                                         double* d_alpha, double* d_beta, int* d_info, double* d_norm, double tol,
+                                         void* stream) {
+  if (!op || !d_state || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || target < 1)
+    return fail(RPM_E_INVALID, "rpm_lanczos_basis_noalloc: bad arguments");
+  const long long n = op->n;
+  const int count = static_cast<int>(target < n ? target : n);
+  if (count > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
+  const size_t need = basis_layout(nullptr, n, count, 1, apply_units(op)).bytes;
+  if (need > kNoAllocScratchBytes)
+    return fail(RPM_E_INVALID,
+                "rpm_lanczos_basis_noalloc: this basis (N = %lld, %d vectors) needs %zu bytes of scratch, above the "
+                "%zu-byte limit; use rpm_lanczos_basis with a workspace",
+                n, count, need, kNoAllocScratchBytes);
+  void* ws = nullptr;
+  const int rc = scratch_address(&ws);
+  if (rc) return rc;
+  // the counters and control blocks that must start at zero are reset by k_basis_start on every call
+  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
+                     static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
+                     d_norm, tol, ws, kNoAllocScratchBytes, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_states, long long ld,
