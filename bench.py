@@ -283,10 +283,12 @@ def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0)
     pairs = batch.pair_matrices(base, [hs[i // samples] for i in range(len(cells))])
     states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, s) for _, s in cells])
     bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
-    res = bp.propagate(states, times, pairs=pairs, progress_callback=progress)
-    yields = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
+    res = bp.propagate(states, times, pairs=pairs, progress_callback=progress,
+                       yield_weights=core.yield_weight_table(times, cfg.lifetime_us))
+    yields = np.array([r.device_yields[0] for r in res])       # summed on the device
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
+    host = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
     diff = yields - np.array([archived[c] for c in cells])
     horizon = float(times[-1])
     per_traj = dt / len(cells)
@@ -296,6 +298,7 @@ def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0)
             "trajectory_us_per_s": len(cells) * horizon / dt, "seconds_per_trajectory": per_traj,
             "matvecs": int(sum(r.stats.matvecs for r in res)), "rounds": bp.rounds,
             "max_abs_yield_minus_archived": float(np.abs(diff).max()),
+            "max_abs_device_minus_host_yield": float(np.abs(yields - host).max()),
             "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / per_traj}
 
 
@@ -453,8 +456,9 @@ def gradcheck_probe(theta: float = 60.0, phi: float = 30.0, field_uT: float = 50
         params += [(theta + dt_, phi + dp, field_uT + df), (theta - dt_, phi - dp, field_uT - df)]
     pairs = batch.pair_matrices(base, [build(t, 1e-3 * f, p) for t, p, f in params])
     res = bp.propagate(np.repeat(state, len(params), axis=0), times, pairs=pairs,
-                       schedules=[center[0].schedule] * len(params))
-    y = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
+                       schedules=[center[0].schedule] * len(params),
+                       yield_weights=core.yield_weight_table(times, cfg.lifetime_us))
+    y = np.array([r.device_yields[0] for r in res])
     fd = np.array([(y[2 * k] - y[2 * k + 1]) / (2 * h) for k, h in enumerate((h_angle, h_angle, h_field))])
     g = np.asarray(out.field_gradient[0], dtype=float)
     scale = 1e-3 * np.abs(fd).max()

@@ -32,6 +32,8 @@
  *   rpm_krylov_tasks           _candidate / evaluate_projected         R/rpm_statevector_engine.py:690-715,531-538
  *   rpm_singlet_gram_batch     SingletPopulationObservable.project_basis R/rpm_statevector_engine.py:521-529
  *   rpm_krylov_combine_batch   endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
+ *   rpm_yield_accumulate       integrate_yield (Simpson) + coarse yield + propagation estimate
+ *                              R/rpm_quantum_needle_jd_sweeps.py:418-420, R/rpm_full14_quadrupole_experiment.py:179-183
  *
  * Segmented propagation and adjoint gradients:
  *   rpm_axis_apply             local_unitary_apply                     R/poster_reproduction/core.py:285-292
@@ -187,6 +189,23 @@ int rpm_krylov_tasks(const void* d_tasks, int ntasks, const double* d_alpha, con
                      int ab_stride, const double* d_norm, const void* d_P, long long p_stride,
                      double safety, double* d_v0, double* d_v1, void* d_coeffs, int m_stride,
                      void* stream);
+
+/*
+ * Singlet yield accumulated on the device as the outputs are produced (R/rpm_quantum_needle_jd_sweeps.py:418-420,
+ * Phi = simpson(exp(-t/tau) P_S / tau, t), and the coarse-grid yield and propagation estimate of
+ * R/rpm_full14_quadrupole_experiment.py:179-183).  For each trajectory b < batch, d_spans[3b..3b+2] =
+ * {row0, count, w0} selects its outputs P_i = d_values[row0 + i] (point errors d_point_err[row0 + i],
+ * nullable = 0) and weight rows W_i = d_weights[2 (w0 + i) .. +1] = (fine, coarse), i < count, and
+ *   d_phi[3b]   += sum_i W_i.fine P_i
+ *   d_phi[3b+1] += sum_i W_i.coarse P_i
+ *   d_phi[3b+2] += sum_i W_i.fine min(1, 2 e_i + e_i^2),  e_i = d_acc[b] (nullable = 0) + point error.
+ * The weights are the Simpson weights of the output grid times exp(-t/tau)/tau (coarse: those of the
+ * coarse sub-grid, 0 elsewhere).  The sum order depends only on the trajectory's own outputs.  Rows or
+ * weights beyond nvalues / nweights contribute nothing.
+ */
+int rpm_yield_accumulate(const double* d_values, const double* d_point_err, long long nvalues,
+                         const long long* d_spans, int batch, const double* d_weights, long long nweights,
+                         const double* d_acc, double* d_phi, void* stream);
 
 /* Batched rpm_singlet_gram: P_b (k_b x k_b at d_P + b*m*m) of the first k_b = d_k[b]
  * vectors of basis b (d_basis + b*m*N); k_b <= 0 skips b. */

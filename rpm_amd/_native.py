@@ -80,6 +80,7 @@ SIGNATURES = {
     "rpm_adjoint_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
     "rpm_lanczos_basis_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                                  _vp]),
+    "rpm_yield_accumulate": (ctypes.c_int, [_vp, _vp, _ll, _vp, ctypes.c_int, _vp, _ll, _vp, _vp, _vp]),
     "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,
                                          _c_double_p, _vp, _c_int_p, ctypes.c_int, _vp, _c_int_p, ctypes.c_int,

@@ -330,7 +330,7 @@ class FieldYield:
     def forward(self, thetas_deg, phis_deg, fields_uT, pulse_amplitudes_uT=None) -> np.ndarray:
         """Yields (B,); keeps what the backward sweep needs."""
         from .batch import pair_matrices
-        from .quadrature import integrate_yield
+        from .quadrature import yield_weight_table
         from .segments import Segment, field_pulse_segments, propagate_segments_batched
 
         times = self.times
@@ -341,9 +341,11 @@ class FieldYield:
         prop = self.propagator
         budget = prop.state_error_budget
         if pulse_amplitudes_uT is None:
-            res = prop.propagate(self.initial_states, times, pairs=pairs)
+            res = prop.propagate(self.initial_states, times, pairs=pairs,
+                                 yield_weights=yield_weight_table(times, self.tau))
             final = prop.final_states
             pops = np.stack([r.observable_values for r in res])
+            yields = np.array([r.device_yields[0] for r in res])
             grid, w = sweep_grid(times, self.tau)
             seg_pairs = pairs[None]
             seg_idx = np.zeros(grid.size - 1, dtype=np.int32)
@@ -359,7 +361,8 @@ class FieldYield:
             segs = [Segment(per_b[0][i].end_us, np.concatenate([per_b[b][i].pairs for b in range(B)]))
                     for i in range(len(per_b[0]))]
             try:
-                out = propagate_segments_batched(prop, self.initial_states, times, segs, budget)
+                out = propagate_segments_batched(prop, self.initial_states, times, segs, budget,
+                                                 yield_weights=yield_weight_table(times, self.tau))
             finally:
                 prop.state_error_budget = budget
             final = out.final_states
@@ -370,7 +373,7 @@ class FieldYield:
             seg_pairs = np.stack([sg.pairs for sg in segs])
             first = 1 if t0 > 0.0 else 0
             pulse_slots = list(range(first, first + amps.shape[1]))
-        yields = np.array([integrate_yield(times, pops[b], self.tau) for b in range(B)])
+            yields = out.yields[:, 0]
         dH = np.stack([field_parameter_derivatives(float(t), float(f), float(p))
                        for t, p, f in zip(thetas_deg, phis_deg, fields_uT)])
         # a new record per call: earlier records stay valid for their own backward()

@@ -32,7 +32,7 @@ import numpy as np
 
 from . import _native as nat
 from .engine import (
-    CompiledFullPairHamiltonian, PropagationResult, PropagationStats, _pair_form,
+    CompiledFullPairHamiltonian, PropagationResult, PropagationStats, YieldAccumulator, _pair_form,
     state_error_to_population_bound,
 )
 
@@ -241,7 +241,7 @@ class BatchedLanczosPropagator:
             while stop < nt and times[stop] <= step_end + 1e-13:
                 stop += 1
             if stop > cursor:
-                vals, perr = yield (_OUT, (selected, low_arg, times[cursor:stop] - now))
+                vals, perr = yield (_OUT, (selected, low_arg, times[cursor:stop] - now, cursor, acc_err))
                 values[cursor:stop] = vals
                 errors[cursor:stop] = acc_err + perr
                 bounds[cursor:stop] = [state_error_to_population_bound(e) for e in errors[cursor:stop]]
@@ -263,7 +263,7 @@ class BatchedLanczosPropagator:
     # -- driver -------------------------------------------------------------
 
     def propagate(self, initial_states, times_us: Iterable[float], *, pairs=None, schedules=None,
-                  progress_callback: Callable | None = None) -> list[PropagationResult]:
+                  progress_callback: Callable | None = None, yield_weights=None) -> list[PropagationResult]:
         """Propagate B trajectories; returns one ``PropagationResult`` per trajectory.
 
         ``initial_states``: (B, N) NumPy array or complex128 GPU tensor (copied).
@@ -274,6 +274,11 @@ class BatchedLanczosPropagator:
         recorded schedules to replay (entries may be None).
         ``progress_callback(b, now, accepted, rejected, matvecs, acc_err)``
         runs after each accepted step of trajectory b.
+        ``yield_weights``: optional (nt, 2) fine and coarse yield weights of the
+        grid (``quadrature.yield_weight_table``), shared, or one table per
+        trajectory; each trajectory's Phi_S, coarse Phi_S and propagation
+        estimate are then summed on the device as its outputs are produced and
+        returned in ``PropagationResult.device_yields``.
         """
         torch = nat.require_gpu()
         lib = nat.load()
@@ -306,6 +311,15 @@ class BatchedLanczosPropagator:
             d_pairs = torch.from_numpy(np.ascontiguousarray(p.reshape(B, 16))).to(dev)
         if schedules is not None and len(schedules) != B:
             raise ValueError("schedules must have one entry per trajectory")
+        yacc = None
+        if yield_weights is not None:
+            tables = list(yield_weights) if isinstance(yield_weights, (list, tuple)) else [yield_weights] * B
+            if len(tables) != B:
+                raise ValueError("yield_weights must be one table or one per trajectory")
+            for b, tab in enumerate(tables):
+                if np.shape(tab) != (traj_times[b].size, 2):
+                    raise ValueError("each yield weight table must have shape (len(times), 2)")
+            yacc = YieldAccumulator(tables)
 
         V = torch.empty((B, m, n), dtype=torch.complex128, device=dev)
         alpha = torch.zeros((B, m), dtype=torch.float64, device=dev)
@@ -364,6 +378,11 @@ class BatchedLanczosPropagator:
                     nv = norms()
                     nat.check(lib.rpm_singlet_population(vp(nat.ptr(states)), n // 4, B, n, vp(nat.ptr(pop_out)),
                                                          vp(nat.ptr(red_ws)), red_ws.numel(), sp))
+                    if yacc is not None:
+                        cnt = np.zeros(B, dtype=np.int64)
+                        for b in group:
+                            cnt[b] = 1 if traj_times[b][0] == 0.0 else 0
+                        yacc.add(pop_out, None, np.arange(B), cnt, np.zeros(B, dtype=np.int64))
                     pv = pop_out.cpu().numpy()
                     for b in group:
                         resume(b, (abs(float(nv[b]) - 1.0), float(pv[b])))
@@ -400,13 +419,18 @@ class BatchedLanczosPropagator:
                 elif phase == _OUT:
                     kb = np.zeros(B, dtype=np.int32)
                     tasks, spans = [], {}
+                    y_rows = np.zeros(B, dtype=np.int64)
+                    y_cnt = np.zeros(B, dtype=np.int64)
+                    y_grid = np.zeros(B, dtype=np.int64)
+                    y_acc = np.zeros(B)
                     for b in group:
-                        sel, low_arg, offs = pending[b][1]
+                        sel, low_arg, offs, cursor, acc_b = pending[b][1]
                         kb[b] = sel
                         start = len(tasks)
                         for t in offs:
                             tasks.append((b, sel, low_arg, len(tasks), float(t), 1, 0))
                         spans[b] = (start, len(tasks))
+                        y_rows[b], y_cnt[b], y_grid[b], y_acc[b] = start, len(tasks) - start, cursor, acc_b
                     d_kb = torch.from_numpy(kb).to(dev)
                     nat.check(lib.rpm_singlet_gram_batch(vp(nat.ptr(V)), m, vp(nat.ptr(d_kb)), B, n // 4,
                                                          vp(nat.ptr(P)), vp(nat.ptr(gram_ws)), gram_ws.numel(), sp))
@@ -414,6 +438,8 @@ class BatchedLanczosPropagator:
                     vals = torch.empty(len(ta), dtype=torch.float64, device=dev)
                     perr = torch.empty(len(ta), dtype=torch.float64, device=dev)
                     run_tasks(ta, True, vals, perr, None)
+                    if yacc is not None:
+                        yacc.add(vals, perr, y_rows, y_cnt, y_grid, y_acc)
                     vh = vals.cpu().numpy()
                     ph_ = perr.cpu().numpy()
                     for b in group:
@@ -436,10 +462,13 @@ class BatchedLanczosPropagator:
                         resume(b, abs(float(nv[b]) - 1.0))
 
         self.final_states = states
+        phi = None if yacc is None else yacc.result()
         out = []
         for b in range(B):
             values, bounds, errors, stats, record = results[b]
-            out.append(PropagationResult(traj_times[b], values, bounds, errors, None, None, stats, record, states[b]))
+            dy = None if phi is None else tuple(float(v) for v in phi[b])
+            out.append(PropagationResult(traj_times[b], values, bounds, errors, None, None, stats, record, states[b],
+                                         dy))
         return out
 
 
@@ -455,7 +484,7 @@ def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, ph
     is ``yields.mean(axis=0)``. ``batch_size="auto"`` picks the batch by timing the first
     trajectories over a short prefix (``autotune.propagation_batch``).
     """
-    from .quadrature import integrate_yield
+    from .quadrature import yield_weight_table
 
     states = np.asarray(states) if not nat.is_torch(states) else states
     thetas = np.atleast_1d(np.asarray(thetas_deg, dtype=float))
@@ -481,14 +510,15 @@ def directional_yields(base, hamiltonian_for, propagator, states, thetas_deg, ph
         idx = [s for s, _, _ in lead]
         batch_size = autotune.propagation_batch(propagator, states[idx], times_us, electron_matrices(lead))["batch"]
     out = np.empty((S, D, F))
+    weights = yield_weight_table(times_us, lifetime_us)     # Phi_S summed on the device
     for c0 in range(0, len(combos), max(1, int(batch_size))):
         chunk = combos[c0:c0 + max(1, int(batch_size))]
         pairs = electron_matrices(chunk)
         idx = [s for s, _, _ in chunk]
         x0 = states[idx] if not nat.is_torch(states) else states[idx]
-        results = propagator.propagate(x0, times_us, pairs=pairs)
+        results = propagator.propagate(x0, times_us, pairs=pairs, yield_weights=weights)
         for (s, d, f), r in zip(chunk, results):
-            out[s, d, f] = integrate_yield(r.times_us, r.observable_values, lifetime_us)
+            out[s, d, f] = r.device_yields[0]
         if progress_callback is not None:
             progress_callback(min(c0 + len(chunk), len(combos)), len(combos))
     return out

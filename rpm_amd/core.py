@@ -23,7 +23,9 @@ from .model import (
     ALL_NUCLEI, GAMMA_E_MHZ_PER_MT, NUCLEAR_RADICALS, QUADRUPOLE_TENSORS_MT, TWOPI, field_vector_mT,
     spin_operators,
 )
-from .quadrature import integrate_singlet_yield, integrate_yield, simpson_weights, yield_weights  # noqa: F401  (re-exported)
+from .quadrature import (  # noqa: F401  (re-exported)
+    integrate_singlet_yield, integrate_yield, simpson_weights, yield_weight_table, yield_weights,
+)
 
 MASTER_SEED = 20260731
 R2_D_PAPER_UT = -2783.0 / (2.0 ** 3)
@@ -225,17 +227,16 @@ def propagate_trajectory(hamiltonian, initial_state, config: SweepConfig, *, pro
     tau = config.lifetime_us if lifetime_us is None else lifetime_us
     prop = make_propagator(hamiltonian, config) if propagator is None else propagator
     grid = config.time_grid_us if times_us is None else np.asarray(times_us, dtype=float)
+    ids = coarse_grid_indices(config) if times_us is None else None
+    # the yield integrals are summed on the device as the populations are produced
     res = prop.propagate(initial_state, grid, SingletPopulationObservable(hamiltonian.dimensions),
-                         progress_callback=progress_callback, schedule=schedule)
-    fine = integrate_yield(grid, res.observable_values, tau)
+                         progress_callback=progress_callback, schedule=schedule,
+                         yield_weights=yield_weight_table(grid, tau, ids))
+    fine, coarse, estimate = res.device_yields
     out = {"result": res, "yield": fine, "times_us": grid, "singlet_population": res.observable_values,
            "population_error_estimate": res.observable_error_bounds,
            "state_error_estimates": res.state_error_estimates, "final_state": res.final_state}
     if times_us is None:
-        ids = coarse_grid_indices(config)
-        coarse = integrate_yield(grid[ids], res.observable_values[ids], tau)
-        w = simpson_weights(grid) * (np.exp(-grid / tau) / tau)
-        out.update(coarse_yield=coarse, quadrature_error=abs(fine - coarse),
-                   yield_propagation_estimate=float(np.dot(w, res.observable_error_bounds)))
+        out.update(coarse_yield=coarse, quadrature_error=abs(fine - coarse), yield_propagation_estimate=estimate)
     out["initial_singlet"] = singlet_population(initial_state, hamiltonian.dimensions)
     return out

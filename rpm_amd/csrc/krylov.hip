@@ -286,6 +286,48 @@ __global__ void __launch_bounds__(128) k_tasks(const KrylovTask* __restrict__ ta
   }
 }
 
+// Singlet-yield accumulation.  Trajectory b's outputs of one call are rows row0 .. row0+count-1 of
+// values / point_err and weight rows w0 .. w0+count-1 of the (fine, coarse) table; one block per
+// trajectory, thread t taking outputs t, t+256, ... and a fixed-shape block sum, so the order of the
+// additions depends only on the trajectory's own outputs (batched and single runs agree bitwise).
+constexpr int kYieldThreads = 256;
+
+__global__ void __launch_bounds__(kYieldThreads) k_yield_accumulate(const double* __restrict__ values,
+                                                                  const double* __restrict__ point_err,
+                                                                  long long nvalues,
+                                                                  const long long* __restrict__ spans,
+                                                                  const double* __restrict__ weights,
+                                                                  long long nweights, const double* __restrict__ acc,
+                                                                  double* __restrict__ phi) {
+  __shared__ double sm[kYieldThreads / kWave];
+  const int b = blockIdx.x;
+  const long long row0 = spans[3 * b], w0 = spans[3 * b + 2];
+  long long count = spans[3 * b + 1];
+  // rows and weights outside the buffers contribute nothing (the host checks its own spans)
+  if (row0 < 0 || w0 < 0) count = 0;
+  count = min(count, min(nvalues - row0, nweights - w0));
+  if (count <= 0) return;                       // uniform over the block
+  const double a = acc ? acc[b] : 0.0;
+  double f = 0.0, c = 0.0, e = 0.0;
+  for (long long i = threadIdx.x; i < count; i += kYieldThreads) {
+    const double p = values[row0 + i];
+    const double wf = weights[2 * (w0 + i)], wc = weights[2 * (w0 + i) + 1];
+    const double err = a + (point_err ? point_err[row0 + i] : 0.0);
+    const double bound = fmin(1.0, 2.0 * err + err * err);  // state_error_to_population_bound
+    f += wf * p;
+    c += wc * p;
+    e += wf * bound;
+  }
+  f = block_sum(f, sm);
+  c = block_sum(c, sm);
+  e = block_sum(e, sm);
+  if (threadIdx.x == 0) {
+    phi[3 * b] += f;
+    phi[3 * b + 1] += c;
+    phi[3 * b + 2] += e;
+  }
+}
+
 }  // namespace rpm
 
 using namespace rpm;
@@ -352,6 +394,17 @@ extern "C" int rpm_krylov_tasks(const void* d_tasks, int ntasks, const double* d
                      static_cast<const KrylovTask*>(d_tasks), d_alpha, d_beta, ab_stride, d_norm,
                      static_cast<const c128*>(d_P), p_stride, safety, d_v0, d_v1, static_cast<c128*>(d_coeffs),
                      m_stride);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+extern "C" int rpm_yield_accumulate(const double* d_values, const double* d_point_err, long long nvalues,
+                                    const long long* d_spans, int batch, const double* d_weights, long long nweights,
+                                    const double* d_acc, double* d_phi, void* stream) {
+  if (!d_values || !d_spans || !d_weights || !d_phi || batch < 1 || batch > 65535 || nvalues < 0 || nweights < 0)
+    return fail(RPM_E_INVALID, "rpm_yield_accumulate: bad arguments");
+  hipLaunchKernelGGL(k_yield_accumulate, dim3(batch), dim3(kYieldThreads), 0, static_cast<hipStream_t>(stream),
+                     d_values, d_point_err, nvalues, d_spans, d_weights, nweights, d_acc, d_phi);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

@@ -212,13 +212,17 @@ def ensure_manifest(output: Path, config: ExperimentConfig) -> None:
 def cell_values(task: Task, config: ExperimentConfig, initial: np.ndarray, result, *, elapsed: float,
                 construction: float, initial_singlet: float) -> dict:
     """The checkpoint dictionary of R/rpm_full14_quadrupole_experiment.py:171-194 for one trajectory."""
-    import scipy.integrate
     times = result.times_us
-    fine = core.integrate_yield(times, result.observable_values, config.lifetime_us)
-    ids = core.coarse_grid_indices(config.sweep)
-    coarse = core.integrate_yield(times[ids], result.observable_values[ids], config.lifetime_us)
-    weights = np.exp(-times / config.lifetime_us) / config.lifetime_us
-    estimate = float(scipy.integrate.simpson(weights * result.observable_error_bounds, x=times))
+    if getattr(result, "device_yields", None) is not None:
+        # summed on the device during the propagation (run_cells passes the fine/coarse weight table)
+        fine, coarse, estimate = result.device_yields
+    else:
+        # a result propagated without yield weights: the same three integrals of its host arrays
+        ids = core.coarse_grid_indices(config.sweep)
+        table = core.yield_weight_table(times, config.lifetime_us, ids)
+        fine = float(np.dot(table[:, 0], result.observable_values))
+        coarse = float(np.dot(table[:, 1], result.observable_values))
+        estimate = float(np.dot(table[:, 0], result.observable_error_bounds))
     stats = asdict(result.stats)
     metadata = {"task": asdict(task), "configuration_hash": config.digest, "elapsed_seconds": elapsed,
                 "construction_seconds": construction, "host": socket.gethostname(), "pid": os.getpid(),
@@ -253,6 +257,7 @@ def run_cells(config: ExperimentConfig, output: Path, todo=None, *, batch_size: 
     dims = config.dimensions()
     cfg = config.sweep
     grid = cfg.time_grid_us
+    weights = core.yield_weight_table(grid, config.lifetime_us, core.coarse_grid_indices(cfg))
     written = []
     for scale in SCALES:
         group = [t for t in pending if t.scale == scale]
@@ -270,7 +275,7 @@ def run_cells(config: ExperimentConfig, output: Path, todo=None, *, batch_size: 
             pairs = pair_matrices(base, hs)
             init = [core.random_phase_initial_state(dims, config.seed, t.sample_id) for t in chunk]
             construction = time.monotonic() - t0 + (time.monotonic() - start if i == 0 else 0.0)
-            res = prop.propagate(np.stack(init), grid, pairs=pairs)
+            res = prop.propagate(np.stack(init), grid, pairs=pairs, yield_weights=weights)
             elapsed = time.monotonic() - t0
             for t, x0, r in zip(chunk, init, res):
                 # the grid starts at t = 0, so the first population is the initial singlet population

@@ -458,6 +458,58 @@ def device_norm(x, out=None):
     return out
 
 
+class YieldAccumulator:
+    """Device weight tables and (Phi_S, coarse Phi_S, estimate) accumulators of B trajectories.
+
+    ``tables``: one (nt_b, 2) array per trajectory (``quadrature.yield_weight_table``); trajectories
+    that share a table object share its device copy.  ``add`` runs ``rpm_yield_accumulate`` on the
+    current stream for the outputs a propagation step has just written on the device.
+    """
+
+    def __init__(self, tables):
+        torch = nat.require_gpu()
+        uniq, index, offsets = [], {}, []
+        total = 0
+        for t in tables:
+            key = id(t)
+            if key not in index:
+                arr = np.ascontiguousarray(np.asarray(t, dtype=float))
+                if arr.ndim != 2 or arr.shape[1] != 2:
+                    raise ValueError("yield weight tables must have shape (len(times), 2)")
+                index[key] = total
+                uniq.append(arr)
+                total += arr.shape[0]
+            offsets.append(index[key])
+        self.sizes = np.array([np.asarray(t).shape[0] for t in tables], dtype=np.int64)
+        self.offsets = np.array(offsets, dtype=np.int64)
+        self.weights = torch.from_numpy(np.concatenate(uniq)).to(nat.device())
+        self.phi = torch.zeros((len(tables), 3), dtype=torch.float64, device=nat.device())
+        self.batch = len(tables)
+
+    def add(self, values, point_err, rows, counts, grid_index, acc=None):
+        """Accumulate outputs rows[b] .. rows[b] + counts[b] - 1 of ``values`` (device) at grid points
+        grid_index[b] ...; ``acc``: accumulated state error per trajectory before the step."""
+        torch = nat.require_gpu()
+        counts = np.asarray(counts, dtype=np.int64)
+        grid_index = np.asarray(grid_index, dtype=np.int64)
+        if np.any((counts > 0) & (grid_index + counts > self.sizes)):
+            raise ValueError("yield outputs beyond the weight table")
+        spans = np.ascontiguousarray(np.stack([np.asarray(rows, dtype=np.int64), counts, self.offsets + grid_index],
+                                              axis=1))
+        dev = values.device
+        d_sp = torch.from_numpy(spans).to(dev)
+        d_acc = None if acc is None else torch.from_numpy(np.ascontiguousarray(acc, dtype=float)).to(dev)
+        vp = ctypes.c_void_p
+        nat.check(nat.load().rpm_yield_accumulate(
+            vp(nat.ptr(values)), vp(nat.ptr(point_err) if point_err is not None else None), values.numel(),
+            vp(nat.ptr(d_sp)), self.batch, vp(nat.ptr(self.weights)), self.weights.shape[0],
+            vp(nat.ptr(d_acc) if d_acc is not None else None), vp(nat.ptr(self.phi)),
+            vp(nat.stream_ptr())))
+
+    def result(self) -> np.ndarray:
+        return self.phi.cpu().numpy()
+
+
 def device_singlet_population(x, batch=1):
     torch = nat.require_gpu()
     n = x.numel() // batch
@@ -560,6 +612,9 @@ class PropagationResult:
     stats: PropagationStats
     schedule: tuple = field(default=())
     final_state_device: object = None
+    # extension: (Phi_S, coarse Phi_S, propagation estimate) accumulated on the device when yield
+    # weights were given (rpm_yield_accumulate)
+    device_yields: tuple | None = None
 
 
 class _KrylovWorkspace:
@@ -671,7 +726,8 @@ class AdaptiveLanczosPropagator:
     # -- propagate ------------------------------------------------------
 
     def propagate(self, initial_state, times_us: Iterable[float], observable: Callable, *, store_states: bool = False,
-                  progress_callback: Callable | None = None, schedule=None) -> PropagationResult:
+                  progress_callback: Callable | None = None, schedule=None,
+                  yield_weights=None) -> PropagationResult:
         """Rules of R/rpm_statevector_engine.py:717-933.
 
         ``schedule`` (extension): a recorded step schedule -- one
@@ -679,6 +735,13 @@ class AdaptiveLanczosPropagator:
         attempt, as returned in ``PropagationResult.schedule`` -- whose
         trial steps, accept/reject decisions and selected dimensions are replayed
         instead of being re-decided from this run's error estimates.
+
+        ``yield_weights`` (extension): (len(times_us), 2) fine and coarse yield
+        weights (``quadrature.yield_weight_table``).  The singlet yield, the
+        coarse-grid yield and the propagation estimate are then summed on the
+        device as the outputs are produced (``rpm_yield_accumulate``) and
+        returned in ``PropagationResult.device_yields``.  Needs this
+        Hamiltonian's ``SingletPopulationObservable`` and ``store_states=False``.
         """
         torch = nat.require_gpu()
         times = np.asarray(tuple(float(t) for t in times_us), dtype=float)
@@ -697,6 +760,14 @@ class AdaptiveLanczosPropagator:
         projected = (not store_states) and callable(project) and callable(evaluate)
         device_projected = projected and ours and tuple(observable.dimensions) == tuple(h.dimensions)
         replay = list(schedule) if schedule is not None else None
+        yacc = None
+        if yield_weights is not None:
+            if not device_projected:
+                raise ValueError("Device yields need this Hamiltonian's SingletPopulationObservable "
+                                 "and store_states=False.")
+            if np.shape(yield_weights) != (times.size, 2):
+                raise ValueError("yield_weights must have shape (len(times_us), 2).")
+            yacc = YieldAccumulator([yield_weights])
 
         nt = times.size
         values = np.empty(nt)
@@ -716,7 +787,12 @@ class AdaptiveLanczosPropagator:
             return float(observable(vec_dev.cpu().numpy()))
 
         if times[0] == 0.0:
-            values[0] = observe(state)
+            if yacc is not None:
+                p0 = device_singlet_population(state)
+                yacc.add(p0, None, [0], [1], [0])
+                values[0] = float(p0.item())
+            else:
+                values[0] = observe(state)
             bounds[0] = errors[0] = 0.0
             if stored is not None:
                 stored[0] = state.cpu().numpy()
@@ -812,6 +888,8 @@ class AdaptiveLanczosPropagator:
                     k, ctypes.c_void_p(nat.ptr(offs)), float(self.safety_factor), ctypes.c_void_p(nat.ptr(vals)),
                     ctypes.c_void_p(nat.ptr(perr)), ctypes.c_void_p(nat.ptr(coeffs) if need_coeffs else None),
                     ctypes.c_void_p(nat.ptr(ws.scratch)), ws.scratch.numel(), ctypes.c_void_p(nat.stream_ptr())))
+                if yacc is not None:
+                    yacc.add(vals, perr, [0], [k], [cursor], [acc_err])
                 perr_h = perr.cpu().numpy()
                 if device_projected:
                     values[cursor:stop] = vals.cpu().numpy()
@@ -853,5 +931,6 @@ class AdaptiveLanczosPropagator:
 
         stats = PropagationStats(accepted, rejected, h.matvec_count - mv0, max_dim, min(used) if used else 0.0,
                                  max(used) if used else 0.0, acc_err, max_norm_err)
+        dy = None if yacc is None else tuple(float(v) for v in yacc.result()[0])
         return PropagationResult(times, values, bounds, errors, stored, state.cpu().numpy(), stats, tuple(record),
-                                 state)
+                                 state, dy)

@@ -56,6 +56,23 @@ def yield_weights(times_us, lifetime_us) -> np.ndarray:
     return simpson_weights(t) * (np.exp(-t / lifetime_us) / lifetime_us)
 
 
+def yield_weight_table(times_us, lifetime_us, coarse_indices=None) -> np.ndarray:
+    """(n, 2) weights for the device yield accumulation (``rpm_yield_accumulate``).
+
+    Column 0: ``yield_weights`` of the grid (Phi_S, and with the population bounds the propagation
+    estimate). Column 1: the same rule on the coarse sub-grid ``times_us[coarse_indices]``, zero at
+    the other points (R/rpm_quantum_needle_jd_sweeps.py:423-431 and the coarse yield of
+    R/rpm_full14_quadrupole_experiment.py:179-183); all zero without coarse indices.
+    """
+    t = np.asarray(times_us, dtype=float)
+    table = np.zeros((t.size, 2))
+    table[:, 0] = yield_weights(t, lifetime_us)
+    if coarse_indices is not None:
+        idx = np.asarray(coarse_indices, dtype=int)
+        table[idx, 1] = yield_weights(t[idx], lifetime_us)
+    return table
+
+
 def integrate_yield(times_us, populations, lifetime_us) -> float:
     """Phi_S for one population trace (R/rpm_quantum_needle_jd_sweeps.py:418-420)."""
     return float(np.dot(yield_weights(times_us, lifetime_us), np.asarray(populations, dtype=float)))

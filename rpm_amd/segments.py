@@ -53,6 +53,7 @@ class SegmentedResult:
     matvecs: np.ndarray
     final_states: object             # device (B, N)
     schedules: list = field(default_factory=list)   # per segment, per trajectory
+    yields: np.ndarray | None = None  # (B, 2) Phi_S and coarse Phi_S when yield weights were given
 
 
 # ---------------------------------------------------------------- reference helpers
@@ -184,13 +185,17 @@ def propagate_segments(hamiltonian, initial_state, times_us, segments: Sequence[
 
 
 def propagate_segments_batched(propagator, initial_states, times_us, segments: Sequence[Segment],
-                               state_error_budget: float, *, schedules=None) -> SegmentedResult:
+                               state_error_budget: float, *, schedules=None, yield_weights=None) -> SegmentedResult:
     """A batch of trajectories through the same segment loop with ``BatchedLanczosPropagator``.
 
     Each segment may give per-trajectory electron matrices (field pulses);
     ``propagator.state_error_budget`` is set per segment as in the reference.
     ``schedules``: optional recorded schedules to replay, one list per
     propagated segment with one entry per trajectory (entries may be None).
+    ``yield_weights``: optional (len(times_us), 2) fine/coarse weight table of
+    the whole grid (``quadrature.yield_weight_table``); each segment then sums
+    its slice of the yields on the device, and ``SegmentedResult.yields``
+    holds their totals.
     """
     torch = nat.require_gpu()
     times = np.asarray(times_us, dtype=float)
@@ -209,6 +214,12 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
     from .engine import device_singlet_population
     pops[:, 0] = device_singlet_population(states.reshape(-1), B).cpu().numpy()
     bounds[:, 0] = 0.0
+    table = yields = None
+    if yield_weights is not None:
+        table = np.asarray(yield_weights, dtype=float)
+        if table.shape != (nt, 2):
+            raise ValueError("yield_weights must have shape (len(times_us), 2)")
+        yields = pops[:, :1] * table[0][None, :]         # the t = 0 term
     cursor, current = 1, 0.0
     acc = np.zeros(B)
     accepted = np.zeros(B, dtype=int)
@@ -226,7 +237,14 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
                     pairs = np.broadcast_to(pairs, (B, 4, 4))
             propagator.state_error_budget = state_error_budget * duration / times[-1]
             replay = None if schedules is None else schedules[len(schedule_log)]
-            res = propagator.propagate(states, local, pairs=pairs, schedules=replay)
+            seg_table = None
+            if table is not None:
+                # the segment's outputs are grid points cursor .. stop-1; an added end point weighs nothing
+                seg_table = np.zeros((local.size, 2))
+                seg_table[:requested.size] = table[cursor:stop]
+            res = propagator.propagate(states, local, pairs=pairs, schedules=replay, yield_weights=seg_table)
+            if yields is not None:
+                yields += np.array([r.device_yields[:2] for r in res])
             count = requested.size
             for b, r in enumerate(res):
                 pops[b, cursor:stop] = r.observable_values[:count]
@@ -242,7 +260,7 @@ def propagate_segments_batched(propagator, initial_states, times_us, segments: S
         if seg.unitary is not None and boundary < times[-1] - 1e-12:
             axis, U = seg.unitary
             states = local_unitary_apply(states, dims, axis, U)
-    return SegmentedResult(times, pops, bounds, acc, accepted, rejected, matvecs, states, schedule_log)
+    return SegmentedResult(times, pops, bounds, acc, accepted, rejected, matvecs, states, schedule_log, yields)
 
 
 def field_pulse_segments(base_pair: np.ndarray, direction, amplitudes_uT, t_start_us: float, t_end_us: float,

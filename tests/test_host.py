@@ -284,3 +284,17 @@ def test_fibonacci_sphere_directions():
     assert np.abs(v.mean(axis=0)).max() < 0.01          # no preferred direction
     with pytest.raises(ValueError):
         workloads.fibonacci_sphere(0)
+
+
+def test_yield_weight_table_matches_scipy_on_fine_and_coarse_grids():
+    """Column 0 reproduces simpson(exp(-t/tau) f / tau, t); column 1 the same on the coarse sub-grid."""
+    cfg = pcore.SweepConfig(lifetime_us=1.0, survival_tail_bound=math.exp(-0.6))
+    t = cfg.time_grid_us
+    ids = pcore.coarse_grid_indices(cfg)
+    f = 0.5 + 0.3 * np.cos(7.0 * t) * np.exp(-t)
+    tab = quadrature.yield_weight_table(t, 1.0, ids)
+    g = np.exp(-t) * f
+    assert abs(tab[:, 0] @ f - scipy.integrate.simpson(g, x=t)) < 1e-15
+    assert abs(tab[:, 1] @ f - scipy.integrate.simpson(g[ids], x=t[ids])) < 1e-15
+    assert np.all(tab[np.setdiff1d(np.arange(t.size), ids), 1] == 0.0)
+    assert np.all(quadrature.yield_weight_table(t, 1.0)[:, 1] == 0.0)

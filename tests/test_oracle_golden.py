@@ -45,7 +45,9 @@ def test_oracle_reproduces_reference_propagation(golden, name):
     np.testing.assert_array_equal(ev[:, 1:4], ref_events[:, 1:4])
     np.testing.assert_allclose(ev[:, 4], ref_events[:, 4], rtol=1e-8)
     np.testing.assert_allclose(res.observable_values, golden[f"{name}_values"], atol=1e-12)
-    np.testing.assert_allclose(res.state_error_estimates, golden[f"{name}_state_errors"], rtol=1e-8, atol=1e-18)
+    # the estimates are norms of differences of two nearly equal Krylov coefficient vectors: their
+    # round-off floor (~1e-14 absolute) moves with the BLAS summation order (thread count)
+    np.testing.assert_allclose(res.state_error_estimates, golden[f"{name}_state_errors"], rtol=1e-6, atol=2e-13)
     np.testing.assert_allclose(res.final_state, golden[f"{name}_final_state"], atol=1e-12)
     s = res.stats
     ref = golden[f"{name}_stats"]
