@@ -81,6 +81,9 @@ SIGNATURES = {
     "rpm_lanczos_basis_noalloc": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, ctypes.c_double,
                                                  _vp]),
     "rpm_yield_accumulate": (ctypes.c_int, [_vp, _vp, _ll, _vp, ctypes.c_int, _vp, _ll, _vp, _vp, _vp]),
+    "rpm_lanczos_basis_batch_prefix": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _ll, _vp, ctypes.c_int, _vp,
+                                                      ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp,
+                                                      _vp, _vp, ctypes.c_double, _vp, _sz, _vp]),
     "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,
                                          _c_double_p, _vp, _c_int_p, ctypes.c_int, _vp, _c_int_p, ctypes.c_int,

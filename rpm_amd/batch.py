@@ -127,6 +127,9 @@ class BatchedLanczosPropagator:
         self.breakdown_tolerance = breakdown_tolerance
         self.rounds = 0
         self.final_states = None     # (B, N) device tensor of the last propagate call
+        # matvecs the device made per trajectory in the last call; stats.matvecs counts the reference's
+        # whole-basis calls for the same schedule (bases are built only as far as a candidate needs)
+        self.device_matvecs = None
 
     @classmethod
     def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
@@ -166,6 +169,8 @@ class BatchedLanczosPropagator:
         prefer_max = False
         mn, mx, inc, off = self.minimum_dimension, self.maximum_dimension, self.dimension_increment, \
             self.embedded_offset
+        guess = mx              # basis prefix to build first in prefer-max mode: the last selected dimension
+        device_mv = [0]         # matvecs the device made (the reference's count is in the stats)
 
         while cursor < nt:
             trial = min(step, t_final - now)
@@ -184,15 +189,7 @@ class BatchedLanczosPropagator:
 
                 def ok(dim, e, b=budget_now):
                     return e <= b
-            target = mx if prefer_max else mn
-            size, snorm = yield (_BASIS, target)
-            matvecs += size
-            if snorm == 0.0:
-                raise ValueError("Cannot propagate the zero vector.")
-            sizes = [size]
             budget = self.state_error_budget * trial / max(t_final, trial)
-            selected, row, err = None, None, math.inf
-            exact = size < target
 
             def search(ladder):
                 errs, rows = yield (_CAND, (ladder, off, trial))
@@ -203,27 +200,63 @@ class BatchedLanczosPropagator:
                         return dim, rows[i], e
                 return None, None, e
 
-            if exact:
+            def exact_step(size):
                 _, rows = yield (_CAND, ([size], 0, trial))
-                selected, row, err = size, rows[0], 0.0
+                return size, rows[0], 0.0
+
+            def continue_search(built, ladder, done):
+                """First passing ladder dimension: those the basis holds above `done`, then the basis is
+                continued one ladder point at a time. Returns (selected, row, err, reference basis size)."""
+                e = math.inf
+                while True:
+                    dims = [d for d in ladder if done < d <= built]
+                    if dims:
+                        sel, r, e = yield from search(dims)
+                        if sel is not None:
+                            return sel, r, e, mx
+                    done = built
+                    if built >= mx:
+                        return None, None, e, mx
+                    stop = min([d for d in ladder if d > built] + [mx])
+                    size, _ = yield (_BASIS, (0, stop, True))
+                    device_mv[0] += size - built
+                    if size < stop:     # breakdown after the prefix: exact, as the whole basis would be
+                        sel, r, e = yield from exact_step(size)
+                        return sel, r, e, size
+                    built = size
+
+            # The reference builds a basis of min (or, prefer-max, max) vectors and searches the ladder
+            # (R/rpm_statevector_engine.py:776-846). Here the basis is built in prefixes: up to the
+            # dimension the last step selected (min before prefer-max), continued (never rebuilt) one
+            # ladder point at a time while no candidate passes. The vectors, candidate errors and decisions
+            # are those of the whole-basis calls; only vectors no candidate needs are left unbuilt.
+            first = min(max(guess, mn), mx) if prefer_max else mn
+            size, snorm = yield (_BASIS, (mx, first, False))
+            if snorm == 0.0:
+                raise ValueError("Cannot propagate the zero vector.")
+            device_mv[0] += size
+            selected, row, err = None, None, math.inf
+            exact = size < first
+            if exact:          # breakdown inside the first prefix: the reference's basis breaks down there too
+                sizes = [size]
+                selected, row, err = yield from exact_step(size)
             elif not prefer_max:
+                sizes = [size]
                 errs, rows = yield (_CAND, ([mn], off, trial))
                 err = float(errs[0])
                 if ok(mn, err):
                     selected, row = mn, rows[0]
                 elif mx > mn:
-                    size, snorm = yield (_BASIS, mx)
-                    matvecs += size
-                    sizes.append(size)
+                    # the reference rebuilds a max basis from the same state: the same vectors, continued
                     prefer_max = True
-                    exact = size < mx
-                    if exact:
-                        _, rows = yield (_CAND, ([size], 0, trial))
-                        selected, row, err = size, rows[0], 0.0
-                    else:
-                        selected, row, err = yield from search(self._ladder(mn + inc))
+                    selected, row, err, ref = yield from continue_search(size, self._ladder(mn + inc), mn)
+                    sizes.append(ref)
+                    exact = ref < mx
             else:
-                selected, row, err = yield from search(self._ladder(mn))
+                selected, row, err, ref = yield from continue_search(size, self._ladder(mn), 0)
+                sizes = [ref]
+                exact = ref < mx
+            matvecs += sum(sizes)
             if selected is None or (replay is None and err > budget):
                 rejected += 1
                 record.append((trial, False, 0, tuple(sizes)))
@@ -231,6 +264,7 @@ class BatchedLanczosPropagator:
                 continue
 
             record.append((trial, True, int(selected), tuple(sizes)))
+            guess = mx if exact else int(selected)
             max_dim = max(max_dim, selected)
             if selected > mn:
                 prefer_max = True
@@ -258,7 +292,7 @@ class BatchedLanczosPropagator:
 
         stats = PropagationStats(accepted, rejected, matvecs, max_dim, min(used) if used else 0.0,
                                  max(used) if used else 0.0, acc_err, max_norm_err)
-        return values, bounds, errors, stats, tuple(record)
+        return values, bounds, errors, stats, tuple(record), device_mv[0]
 
     # -- driver -------------------------------------------------------------
 
@@ -387,14 +421,22 @@ class BatchedLanczosPropagator:
                     for b in group:
                         resume(b, (abs(float(nv[b]) - 1.0), float(pv[b])))
                 elif phase == _BASIS:
-                    tg = np.zeros(B, dtype=np.int32)
+                    # targets (0: continue an earlier prefix) then stops, one upload
+                    ts = np.zeros(2 * B, dtype=np.int32)
+                    j0 = m
                     for b in group:
-                        tg[b] = min(int(pending[b][1]), n)
-                    d_tg = torch.from_numpy(tg).to(dev)
-                    nat.check(lib.rpm_lanczos_basis_batch(
-                        vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_tg)), int(tg.max()),
-                        vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)), m,
-                        vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
+                        target, stop, cont = pending[b][1]
+                        ts[B + b] = min(int(stop), n)
+                        if cont:
+                            j0 = min(j0, int(size_h[b]))
+                        else:
+                            ts[b] = min(int(target), n)
+                            j0 = 0
+                    d_ts = torch.from_numpy(ts).to(dev)
+                    nat.check(lib.rpm_lanczos_basis_batch_prefix(
+                        vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_ts)), m, vp(nat.ptr(d_ts) + 4 * B),
+                        j0, int(ts[B:].max()), vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)),
+                        m, vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
                         float(self.breakdown_tolerance), vp(nat.ptr(ws)), ws.numel(), sp))
                     inf_h = info.cpu().numpy()
                     nrm_h = norm.cpu().numpy()
@@ -462,10 +504,11 @@ class BatchedLanczosPropagator:
                         resume(b, abs(float(nv[b]) - 1.0))
 
         self.final_states = states
+        self.device_matvecs = np.array([results[b][5] for b in range(B)], dtype=np.int64)
         phi = None if yacc is None else yacc.result()
         out = []
         for b in range(B):
-            values, bounds, errors, stats, record = results[b]
+            values, bounds, errors, stats, record, _ = results[b]
             dy = None if phi is None else tuple(float(v) for v in phi[b])
             out.append(PropagationResult(traj_times[b], values, bounds, errors, None, None, stats, record, states[b],
                                          dy))

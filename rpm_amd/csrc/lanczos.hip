@@ -93,51 +93,65 @@ __global__ void __launch_bounds__(kThreads) k_finish_sum(const double* __restric
 
 // ---------------------------------------------------------------- basis kernels
 
-// Trajectory b takes part in the step of Lanczos index j that still needs
-// `need` more vectors after j: active && j + need < target.
+// Trajectory b takes part in the step of Lanczos index j (inside its window of this call) that
+// still needs `need` more vectors after j: j + need < target.
 __device__ inline bool basis_live(const BasisCtl* ctl, int b, int j, int need) {
   const BasisCtl& c = ctl[b];
-  return c.active && j + need < c.target;
+  return basis_step_live(c, j) && j + need < c.target;
 }
 
-// Norm of each starting state; resets the control blocks and the k_lcgs_finish counters.
+// Norm of each starting state and the control blocks of this call. Trajectory b with targets[b] > 0
+// (every trajectory without targets) starts a new basis of eventual size min(targets[b], uniform) and
+// builds its first stops[b] vectors now (all of them without stops); with targets[b] == 0 and
+// stops[b] > 0 it resumes the basis an earlier call left (same workspace) up to stops[b] vectors;
+// otherwise it sits this call out and keeps everything. Resets the k_lcgs_finish counters of new
+// bases.
 __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restrict__ partial, int nblk,
                                                           double* __restrict__ norm_out, BasisCtl* ctl,
                                                           unsigned* __restrict__ counter,
-                                                          const int* __restrict__ targets, int uniform, long long n) {
+                                                          const int* __restrict__ targets,
+                                                          const int* __restrict__ stops, int uniform, long long n) {
   __shared__ double sm[kThreads / kWave];
   const int b = blockIdx.x;
+  const int tg = targets ? targets[b] : uniform;
+  const int st = stops ? stops[b] : 0;
+  BasisCtl& C = ctl[b];
+  if (tg <= 0) {
+    if (threadIdx.x == 0) {
+      if (targets && tg == 0 && st > 0) {
+        C.part = 1;
+        C.begin = C.stop;
+        C.stop = max(C.stop, min(st, C.target));
+      } else {
+        C.part = 0;
+        C.begin = C.stop;     // no step of this call is live for it
+      }
+    }
+    return;
+  }
   double s = 0.0;
   for (int i = threadIdx.x; i < nblk; i += kThreads) s += partial[static_cast<long long>(b) * nblk + i];
   s = block_sum(s, sm);
   if (threadIdx.x == 0) {
     const double nrm = std::sqrt(s);
-    int count = targets ? min(targets[b], uniform) : uniform;
+    int count = min(tg, uniform);
     if (count > n) count = static_cast<int>(n);
+    const int first = st > 0 ? min(st, count) : count;
     counter[b] = 0;
-    ctl[b].nfix = 0;
-    if (count <= 0) {
-      // not part of this call: its norm, basis and tridiagonal stay as they are
-      ctl[b].active = 0;
-      ctl[b].size = 0;
-      ctl[b].target = -1;
-      ctl[b].scale = 0.0;
-      ctl[b].beta_prev = 0.0;
-    } else {
-      norm_out[b] = nrm;
-      const bool live = nrm > 0.0;
-      ctl[b].active = live ? 1 : 0;
-      ctl[b].size = live ? count : 0;
-      ctl[b].target = live ? count : 0;
-      ctl[b].scale = live ? 1.0 / nrm : 0.0;
-      ctl[b].beta_prev = 0.0;
-    }
+    C.nfix = 0;
+    norm_out[b] = nrm;
+    const bool live = nrm > 0.0;
+    C.active = live ? 1 : 0;
+    C.size = live ? count : 0;
+    C.target = live ? count : 0;
+    C.scale = live ? 1.0 / nrm : 0.0;
+    C.beta_prev = 0.0;
+    C.begin = 0;
+    C.stop = live ? first : 0;
+    C.part = 1;
   }
 }
 
-// h[c] = sum_blk partial[c][blk]; optionally alpha[j] = Re h[alpha_col]. With dpart, the last block
-// of each trajectory instead forms dsum[b] = sum_blk dpart[b][blk] (the |y|^2 of the lagged pass),
-// in the order k_lcgs_finish used to.
 __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict__ partial, int nblk, long long pstride,
                                                           c128* __restrict__ h, int hstride, double* __restrict__ alpha,
                                                           int astride, int alpha_col, int j, int need,
@@ -440,10 +454,12 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   }
 }
 
+// info of the trajectories that took part: vectors built so far (size is the breakdown size or the
+// eventual target), no-breakdown flag, explicit corrections.
 __global__ void k_basis_info(const BasisCtl* ctl, int* info, int batch) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < batch && ctl[b].target >= 0) {
-    info[b * 4 + 0] = ctl[b].size;
+  if (b < batch && ctl[b].part) {
+    info[b * 4 + 0] = min(ctl[b].size, ctl[b].stop);
     info[b * 4 + 1] = ctl[b].active;
     info[b * 4 + 2] = ctl[b].nfix;
   }
@@ -676,22 +692,29 @@ void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int 
 }
 
 // Shared implementation of the single and batched basis builders.
+// Steps j_begin <= j < max_stop of the bases of this call (per trajectory: its window from
+// k_basis_start); without stops, every basis is built whole in one call (j_begin 0, max_stop the
+// target).
 int basis_batch(const rpm_operator* op, int batch, const c128* states, long long sld, const int* d_targets,
-                int max_target, const c128* pairs, c128* V, long long vstride, int m, double* alpha, double* beta,
-                int* info, double* norm, double tol, void* ws_ptr, size_t ws_bytes, hipStream_t s) {
+                int max_target, const int* d_stops, int j_begin, int max_stop, const c128* pairs, c128* V,
+                long long vstride, int m, double* alpha, double* beta, int* info, double* norm, double tol,
+                void* ws_ptr, size_t ws_bytes, hipStream_t s) {
   const long long n = op->n;
   if (max_target > m || max_target < 1) return fail(RPM_E_INVALID, "target %d outside 1..%d", max_target, m);
+  if (max_stop > max_target || max_stop < 1 || j_begin < 0 || j_begin >= max_stop)
+    return fail(RPM_E_INVALID, "basis steps %d..%d outside 0..%d", j_begin, max_stop, max_target);
   if (m > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
   BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch, apply_units(op));
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "Lanczos workspace %zu < %zu", ws_bytes, w.bytes);
   const int nchunk = ceil_div(n, kChunk);
-  const int count = static_cast<int>(max_target < n ? max_target : n);
+  const int full = static_cast<int>(max_target < n ? max_target : n);
+  const int count = static_cast<int>(max_stop < n ? max_stop : n);
   hipLaunchKernelGGL(k_sumsq, dim3(nchunk, batch), dim3(kThreads), 0, s, states, n, sld, w.sq, d_targets);
   RPM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, w.counter, d_targets,
-                     count, n);
+                     d_stops, full, n);
   RPM_LAUNCH_CHECK();
-  for (int j = 0; j < count; ++j) {
+  for (int j = j_begin; j < count; ++j) {
     const c128* in = j == 0 ? states : w.w[j % 2];
     c128* out = w.w[(j + 1) % 2];
     // matvec on x_j = s in: out = H x_j and the per-unit partials of alpha~_j = <x_j, H x_j>; the
@@ -725,7 +748,9 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
                        nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
     RPM_LAUNCH_CHECK();
     // alpha_j, beta_j, c_{j+1}; the explicit correction when it is due
-    hipLaunchKernelGGL(k_lcgs_finish, dim3(j == count - 1 ? 1 : kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n,
+    // the last step of a basis forms alpha only (one block); a prefix's last step also forms beta
+    // and the pending correction, so that a later call can resume it
+    hipLaunchKernelGGL(k_lcgs_finish, dim3(j == full - 1 ? 1 : kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n,
                        w.ysum, w.h, m, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter, w.ctl);
     RPM_LAUNCH_CHECK();
   }
@@ -755,7 +780,7 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
   hipStream_t s = static_cast<hipStream_t>(stream);
   BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1, apply_units(op));
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
-  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
+  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr, 0, count, nullptr,
                      static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
                      d_norm, tol, d_ws, ws_bytes, s);
 }
@@ -814,7 +839,7 @@ extern "C" int rpm_lanczos_basis_noalloc(const rpm_operator* op, const void* d_s
   const int rc = scratch_address(&ws);
   if (rc) return rc;
   // the counters and control blocks that must start at zero are reset by k_basis_start on every call
-  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr,
+  return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr, 0, count, nullptr,
                      static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
                      d_norm, tol, ws, kNoAllocScratchBytes, static_cast<hipStream_t>(stream));
 }
@@ -828,8 +853,23 @@ extern "C" int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const 
     return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch: bad arguments");
   // the first matvec reads the states and writes the packed work vectors with one stride
   if (ld != op->n) return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch: states must be packed (ld == N)");
-  return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target,
+  return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target, nullptr, 0, max_target,
                      static_cast<const c128*>(d_pairs), static_cast<c128*>(d_basis),
+                     static_cast<long long>(m) * op->n, m, d_alpha, d_beta, d_info, d_norm, tol, d_ws, ws_bytes,
+                     static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rpm_lanczos_basis_batch_prefix(const rpm_operator* op, int batch, const void* d_states, long long ld,
+                                              const int* d_targets, int max_target, const int* d_stops, int j_begin,
+                                              int max_stop, const void* d_pairs, void* d_basis, int m,
+                                              double* d_alpha, double* d_beta, int* d_info, double* d_norm,
+                                              double tol, void* d_ws, size_t ws_bytes, void* stream) {
+  if (!op || !d_states || !d_targets || !d_stops || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || !d_ws ||
+      batch < 1 || batch > 65535 || m < 1)
+    return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: bad arguments");
+  if (ld != op->n) return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: states must be packed (ld == N)");
+  return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target, d_stops, j_begin,
+                     max_stop, static_cast<const c128*>(d_pairs), static_cast<c128*>(d_basis),
                      static_cast<long long>(m) * op->n, m, d_alpha, d_beta, d_info, d_norm, tol, d_ws, ws_bytes,
                      static_cast<hipStream_t>(stream));
 }

@@ -49,7 +49,7 @@ __global__ void __launch_bounds__(256) k_pair_apply(PairTables T, const c128* __
   double s = 1.0;
   if (A.ctl) {
     const BasisCtl& c = A.ctl[b];
-    if (!c.active || A.j >= c.target) return;
+    if (!basis_step_live(c, A.j)) return;
     s = c.scale;
   }
   const long long t0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -409,7 +409,7 @@ __global__ void __launch_bounds__(256) k_local_apply(int nterms, long long n,
   double s = 1.0;
   if (A.ctl) {
     const BasisCtl& c = A.ctl[b];
-    if (!c.active || A.j >= c.target) return;
+    if (!basis_step_live(c, A.j)) return;
     s = c.scale;
   }
   const long long i0 = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -72,7 +72,7 @@ __device__ inline bool tile_begin(const ApplyCtl& A, int b, double& s) {
   s = 1.0;
   if (A.ctl) {
     const BasisCtl& c = A.ctl[b];
-    if (!c.active || A.j >= c.target) return false;
+    if (!basis_step_live(c, A.j)) return false;
     s = c.scale;
   }
   return true;

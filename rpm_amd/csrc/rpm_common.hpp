@@ -121,7 +121,16 @@ struct BasisCtl {
   int nfix;         // explicit corrections of the lagged projection applied in this basis
   double scale;     // 1/norm of the vector the next matvec normalises
   double beta_prev; // beta_{j-1} of the Lanczos three-term step (0 for j = 0)
+  int begin;        // Lanczos steps begin <= j < stop run in the current call (a prefix of the
+  int stop;         // basis now, the rest when a later call resumes it)
+  int part;         // 1 if the trajectory takes part in the current call
+  int pad_;
 };
+
+// Step j of the basis runs for this trajectory in the current call.
+__host__ __device__ inline bool basis_step_live(const BasisCtl& c, int j) {
+  return c.active && j < c.target && j >= c.begin && j < c.stop;
+}
 
 // Options of one batched H·psi launch (all nullable / off by default):
 //  * ctl: per-vector control; vector b is skipped when !ctl[b].active or

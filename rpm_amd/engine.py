@@ -668,6 +668,10 @@ class AdaptiveLanczosPropagator:
         self.minimum_step_us = minimum_step_us
         self.breakdown_tolerance = breakdown_tolerance
         self._ws = None
+        self._built = 0
+        # matvecs the device made in the last propagate call; stats.matvecs (and the Hamiltonian's
+        # matvec_count) count the reference's whole-basis calls for the same schedule
+        self.device_matvecs = 0
 
     # -- device pieces ------------------------------------------------
 
@@ -679,20 +683,52 @@ class AdaptiveLanczosPropagator:
             self._ws = _KrylovWorkspace(self.hamiltonian._device_op(), m)
         return self._ws
 
-    def _basis(self, state, target):
-        """Device Lanczos basis; returns (size, alpha, beta) on the host and fills the workspace."""
+    def _basis(self, state, target, stop=None, resume=False):
+        """Device Lanczos basis in prefixes: a new basis of eventual size ``target`` built to ``stop``
+        vectors (all of them by default), or (``resume``) the current one continued to ``stop``.
+        Returns the vectors built."""
+        if stop is None:
+            stop = target
+        torch = nat.require_gpu()
         ws = self._workspace()
         op = self.hamiltonian._device_op()
-        count = min(int(target), state.numel())
+        n = state.numel()
         lib = nat.load()
-        nat.check(lib.rpm_lanczos_basis(
-            ctypes.c_void_p(op.handle), ctypes.c_void_p(nat.ptr(state)), count, ctypes.c_void_p(nat.ptr(ws.V)),
+        cap = min(ws.m, n)
+        ts = np.array([0 if resume else min(int(target), cap), min(int(stop), cap)], dtype=np.int32)
+        d_ts = torch.from_numpy(ts).to(state.device)
+        j0 = self._built if resume else 0
+        nat.check(lib.rpm_lanczos_basis_batch_prefix(
+            ctypes.c_void_p(op.handle), 1, ctypes.c_void_p(nat.ptr(state)), n, ctypes.c_void_p(nat.ptr(d_ts)), ws.m,
+            ctypes.c_void_p(nat.ptr(d_ts) + 4), j0, int(ts[1]), None, ctypes.c_void_p(nat.ptr(ws.V)), ws.m,
             ctypes.c_void_p(nat.ptr(ws.alpha)), ctypes.c_void_p(nat.ptr(ws.beta)), ctypes.c_void_p(nat.ptr(ws.info)),
             ctypes.c_void_p(nat.ptr(ws.norm)), float(self.breakdown_tolerance), ctypes.c_void_p(nat.ptr(ws.ws)),
             ws.ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
         size = int(ws.info[0].item())
-        self.hamiltonian.matvec_count += size
+        self.device_matvecs += size - j0
+        self._built = size
         return size
+
+    def _continue_search(self, state, built, ladder, done, t, ok):
+        """First passing ladder dimension: those the basis holds above ``done``, then the basis is
+        continued one ladder point at a time. Returns (dim, index, err, the reference's basis size)."""
+        mx = self.maximum_dimension
+        err = math.inf
+        while True:
+            dims = [d for d in ladder if done < d <= built]
+            if dims:
+                dim, idx, err = self._search(dims, t, ok)
+                if dim is not None:
+                    return dim, idx, err, mx
+            done = built
+            if built >= mx:
+                return None, None, err, mx
+            stop = min([d for d in ladder if d > built] + [mx])
+            size = self._basis(state, mx, stop, resume=True)
+            if size < stop:     # breakdown after the prefix: exact, as the whole basis would be
+                self._candidates([size], 0, t)
+                return size, 0, 0.0, size
+            built = size
 
     def _candidates(self, dims, offset, t):
         ws = self._workspace()
@@ -800,6 +836,8 @@ class AdaptiveLanczosPropagator:
         t_final = float(times[-1])
         step = min(self.initial_step_us, t_final) if t_final else 0.0
         prefer_max = False
+        guess = self.maximum_dimension   # basis prefix to build first in prefer-max mode
+        self.device_matvecs = 0
         ws = None
         dev = nat.device()
         new_state = torch.empty_like(state)
@@ -819,41 +857,46 @@ class AdaptiveLanczosPropagator:
                     return bool(rec[1]) and int(dim) == int(rec[2])
             else:
                 passes = None
-            target = self.maximum_dimension if prefer_max else self.minimum_dimension
-            size = self._basis(state, target)
+            # The basis is built in prefixes (up to the dimension the last step selected; min before
+            # prefer-max) and continued one ladder point at a time while no candidate passes: the vectors,
+            # candidate errors and decisions are those of the reference's whole-basis calls
+            # (R/rpm_statevector_engine.py:776-846), without the vectors no candidate needs.
+            mn, mx = self.minimum_dimension, self.maximum_dimension
+            first = min(max(guess, mn), mx) if prefer_max else mn
+            size = self._basis(state, mx, first)
             ws = self._workspace()
             snorm = float(ws.norm.item())
             if snorm == 0.0:
                 raise ValueError("Cannot propagate the zero vector.")
-            sizes = [size]
             budget = self.state_error_budget * trial / max(t_final, trial)
             selected, cand_index, err = None, None, math.inf
-            exact = size < target
+            exact = size < first
 
             def ok(dim, e):
                 return passes(dim, e) if passes is not None else e <= budget
 
             if exact:
+                sizes = [size]
                 self._candidates([size], 0, trial)
                 selected, cand_index, err = size, 0, 0.0
             elif not prefer_max:
-                errs = self._candidates([self.minimum_dimension], self.embedded_offset, trial)
+                sizes = [size]
+                errs = self._candidates([mn], self.embedded_offset, trial)
                 err = float(errs[0])
-                if ok(self.minimum_dimension, err):
-                    selected, cand_index = self.minimum_dimension, 0
-                elif self.maximum_dimension > self.minimum_dimension:
-                    size = self._basis(state, self.maximum_dimension)
-                    sizes.append(size)
+                if ok(mn, err):
+                    selected, cand_index = mn, 0
+                elif mx > mn:
+                    # the reference rebuilds a max basis from the same state: the same vectors, continued
                     prefer_max = True
-                    exact = size < self.maximum_dimension
-                    if exact:
-                        self._candidates([size], 0, trial)
-                        selected, cand_index, err = size, 0, 0.0
-                    else:
-                        selected, cand_index, err = self._search(
-                            self._ladder(self.minimum_dimension + self.dimension_increment), trial, ok)
+                    selected, cand_index, err, ref = self._continue_search(
+                        state, size, self._ladder(mn + self.dimension_increment), mn, trial, ok)
+                    sizes.append(ref)
+                    exact = ref < mx
             else:
-                selected, cand_index, err = self._search(self._ladder(self.minimum_dimension), trial, ok)
+                selected, cand_index, err, ref = self._continue_search(state, size, self._ladder(mn), 0, trial, ok)
+                sizes = [ref]
+                exact = ref < mx
+            h.matvec_count += sum(sizes)      # the reference's count for this schedule
             rejected_now = selected is None or (passes is None and err > budget)
             if rejected_now:
                 rejected += 1
@@ -862,6 +905,7 @@ class AdaptiveLanczosPropagator:
                 continue
 
             record.append((trial, True, int(selected), tuple(sizes)))
+            guess = self.maximum_dimension if exact else int(selected)
             max_dim = max(max_dim, selected)
             if selected > self.minimum_dimension:
                 prefer_max = True

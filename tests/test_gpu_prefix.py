@@ -1,0 +1,101 @@
+"""GPU: Lanczos bases built in prefixes (rpm_lanczos_basis_batch_prefix) and the propagators that use them.
+
+* A basis built in pieces (24, then 40, then 64 vectors; with one trajectory starting a new basis
+  while another continues) is bitwise the basis built in one call (R/rpm_statevector_engine.py:651-688).
+* The propagators keep the reference's schedules and values (the oracle comparisons in
+  test_gpu_parity.py run through the same code) while building fewer vectors: the device matvec count
+  is below the reference's count whenever a step selects a dimension below the maximum.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+from rpm_amd import _native as nat  # noqa: E402
+from rpm_amd import batch, core, engine  # noqa: E402
+
+
+def _bufs(B, m, n, dev):
+    return dict(V=torch.zeros((B, m, n), dtype=torch.complex128, device=dev),
+                alpha=torch.zeros((B, m), dtype=torch.float64, device=dev),
+                beta=torch.zeros((B, m), dtype=torch.float64, device=dev),
+                info=torch.zeros((B, 4), dtype=torch.int32, device=dev),
+                norm=torch.zeros(B, dtype=torch.float64, device=dev))
+
+
+def test_prefix_basis_equals_whole_basis_bitwise():
+    print("case: production operator, 3 trajectories, prefixes 24/40/64", flush=True)
+    h = core.build_hamiltonian(90.0, 0.05)
+    lib = nat.load()
+    op = ctypes.c_void_p(h._device_op().handle)
+    n, B, m = h.dimension, 3, 64
+    dev = nat.device()
+    states = torch.from_numpy(np.stack([core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i)
+                                        for i in range(B)])).to(dev)
+    pairs = torch.from_numpy(np.ascontiguousarray(
+        batch.pair_matrices(h, [core.build_hamiltonian(t, 0.05) for t in (0.0, 45.0, 90.0)]).reshape(B, 16))).to(dev)
+    vp = ctypes.c_void_p
+    sp = vp(nat.stream_ptr())
+
+    def ws():
+        return nat.workspace(lib.rpm_lanczos_batch_workspace_bytes(op, m, B))
+
+    whole = _bufs(B, m, n, dev)
+    w1 = ws()
+    tg = torch.tensor([64, 64, 40], dtype=torch.int32, device=dev)
+    nat.check(lib.rpm_lanczos_basis_batch(op, B, vp(nat.ptr(states)), n, vp(nat.ptr(tg)), m, vp(nat.ptr(pairs)),
+                                          vp(nat.ptr(whole["V"])), m, vp(nat.ptr(whole["alpha"])),
+                                          vp(nat.ptr(whole["beta"])), vp(nat.ptr(whole["info"])),
+                                          vp(nat.ptr(whole["norm"])), 1e-14, vp(nat.ptr(w1)), w1.numel(), sp))
+    part = _bufs(B, m, n, dev)
+    w2 = ws()
+
+    def call(targets, stops, j0):
+        t = torch.tensor(list(targets) + list(stops), dtype=torch.int32, device=dev)
+        nat.check(lib.rpm_lanczos_basis_batch_prefix(
+            op, B, vp(nat.ptr(states)), n, vp(nat.ptr(t)), m, vp(nat.ptr(t) + 4 * B), j0, max(stops),
+            vp(nat.ptr(pairs)), vp(nat.ptr(part["V"])), m, vp(nat.ptr(part["alpha"])), vp(nat.ptr(part["beta"])),
+            vp(nat.ptr(part["info"])), vp(nat.ptr(part["norm"])), 1e-14, vp(nat.ptr(w2)), w2.numel(), sp))
+        torch.cuda.synchronize()
+        return part["info"][:, 0].cpu().numpy().copy()
+
+    # trajectories 0 and 1 start now (prefix 24); trajectory 2 sits out
+    assert list(call([64, 64, 0], [24, 24, 0], 0)[:2]) == [24, 24]
+    # 0 continues to 40; 1 sits out; 2 starts a basis of 40 whole
+    got = call([0, 0, 40], [40, 0, 40], 0)
+    assert list(got) == [40, 24, 40]
+    # 0 and 1 continue to 64 (1 from 24)
+    assert list(call([0, 0, 0], [64, 64, 0], 24)) == [64, 64, 40]
+    for b, k in enumerate((64, 64, 40)):
+        assert torch.equal(part["V"][b, :k], whole["V"][b, :k]), b
+        assert torch.equal(part["alpha"][b, :k], whole["alpha"][b, :k]), b
+        assert torch.equal(part["beta"][b, :k - 1], whole["beta"][b, :k - 1]), b
+        assert torch.equal(part["norm"][b], whole["norm"][b])
+    assert torch.equal(part["info"][:, :3], whole["info"][:, :3])
+
+
+def test_propagators_build_fewer_vectors_with_the_same_schedule():
+    print("case: production operator, 0.3 us, single and batched", flush=True)
+    h = core.build_hamiltonian(90.0, 0.05)
+    times = np.arange(241) * 0.00125
+    states = np.stack([core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i) for i in range(2)])
+    kw = dict(state_error_budget=2.4e-5 * 0.3 / 57.56)
+    obs = engine.SingletPopulationObservable(h.dimensions)
+    single = engine.AdaptiveLanczosPropagator(h, **kw)
+    one = single.propagate(states[0], times, obs)
+    bp = batch.BatchedLanczosPropagator(h, **kw)
+    res = bp.propagate(states, times)
+    assert res[0].schedule == one.schedule
+    assert res[0].stats == one.stats
+    np.testing.assert_allclose(res[0].observable_values, one.observable_values, rtol=0, atol=1e-13)
+    sel = [r[2] for r in one.schedule if r[1]]
+    print("selected", sel, "reference matvecs", one.stats.matvecs, "device", single.device_matvecs, flush=True)
+    assert single.device_matvecs == bp.device_matvecs[0]
+    assert single.device_matvecs <= one.stats.matvecs
+    # the reference's count: every prefer-max step builds the maximum basis
+    assert one.stats.matvecs == sum(sum(r[3]) for r in one.schedule)
+    if min(sel[1:]) < 64:
+        assert single.device_matvecs < one.stats.matvecs
