@@ -350,6 +350,7 @@ class FieldYield:
             seg_pairs = pairs[None]
             seg_idx = np.zeros(grid.size - 1, dtype=np.int32)
             pulse_slots = []
+            schedules = [[r.schedule for r in res]]
         else:
             if self.pulse_window is None:
                 raise ValueError("pulse amplitudes need pulse_window_us")
@@ -374,11 +375,13 @@ class FieldYield:
             first = 1 if t0 > 0.0 else 0
             pulse_slots = list(range(first, first + amps.shape[1]))
             yields = out.yields[:, 0]
+            schedules = out.schedules
         dH = np.stack([field_parameter_derivatives(float(t), float(f), float(p))
                        for t, p, f in zip(thetas_deg, phis_deg, fields_uT)])
         # a new record per call: earlier records stay valid for their own backward()
         self._saved = dict(final=final.clone(), grid=grid, w=w, seg_pairs=seg_pairs, seg_idx=seg_idx,
-                           nseg=seg_pairs.shape[0], pulse_slots=pulse_slots, dH=dH, pops=pops)
+                           nseg=seg_pairs.shape[0], pulse_slots=pulse_slots, dH=dH, pops=pops,
+                           schedules=schedules)
         return yields
 
     def saved_state(self):
