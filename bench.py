@@ -5,14 +5,18 @@ production operator (theta = 90 deg, 50 uT, Efimova-Hore D/J = -3, 14N
 quadrupoles on).  value = H·psi per second over the timed steps; the roofline
 object prices the kernel's algorithmic bytes (read psi + write H psi,
 32 B per amplitude = 7,077,888 B per vector) against the 8 TB/s HBM peak.
-The same JSON line carries the other two parts of the metric from short
-probes: "batched" (32 trajectories = 8 field orientations x 4 random-phase
-vectors through the reference's adaptive Lanczos rules, against the plain
-one-trajectory-at-a-time path) and "gradient" (yield plus adjoint gradient
+The same JSON line carries the other two parts of the metric over the whole
+57.56 us horizon (the reference's unit of work is a whole trajectory):
+"batched" (32 whole trajectories of the reference's archived run, 8 field
+angles x 4 random-phase vectors, in one batched propagation, against
+"trajectory", one whole trajectory of the same run on the plain path; yields
+per second and their ratio; yields checked against the archive) and "gradient"
+(8 whole production trajectories: yield plus adjoint (theta, phi, |B|) gradient
 over yield alone), plus "spectral" (the factorized J = D = 0 oracle's 199-angle
-curve on the GPU against the oracle's per-orientation CPU time), "sobol" (16 batched
-Sobol reaction-time cells on the full system) and "trajectory" (one whole 57.56 us
-trajectory of the reference's archived run, against its archived wall time).
+curve on the GPU against the oracle's per-orientation CPU time), "sobol" (16
+batched Sobol reaction-time cells on the full system), "directional" (the
+directional-map composition over a short horizon) and "propagation" (the
+reference's 0.1 us speed probe, priced on the bytes this implementation moves).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
     python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
@@ -98,7 +102,10 @@ print(json.dumps(out))
                NUMEXPR_NUM_THREADS="1")
     res = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, check=True)
     out = json.loads(res.stdout.strip().splitlines()[-1])
-    workers = max(1, min(16, os.cpu_count() or 1))
+    # one worker per CPU this process may run on (its affinity set), at most 16: the GPU box gives one
+    # GPU's run a share of 16 CPUs whatever os.cpu_count() reports
+    affinity = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    workers = max(1, min(16, affinity))
     worker = r"""
 import json, sys, time
 sys.path.insert(0, %r)
@@ -122,6 +129,8 @@ print(json.dumps({"n": n, "dt": time.perf_counter() - t0}))
         rate += r["n"] / r["dt"]
     out["all_core_hpsi_per_s"] = rate
     out["all_core_workers"] = workers
+    out["host_cpu_count"] = os.cpu_count()
+    out["host_affinity_cpus"] = affinity
     out["kind"] = "port"
     return out
 
@@ -151,14 +160,16 @@ def lanczos_algorithmic_bytes(schedule, n: int) -> int:
     return total
 
 
-def lanczos_device_bytes(schedule, n: int) -> int:
+def lanczos_device_bytes(schedule, n: int, built=None) -> int:
     """HBM bytes this implementation moves for the same schedule (its lagged one-pass orthogonalisation):
     per Lanczos step j of a basis, 16 N (2 [matvec: in, H x_j] + j + 4 [one pass over V[0..j), x_j and
     H x_j read, v_j and y written]); per accepted step the same endpoint and singlet terms as
-    lanczos_algorithmic_bytes. The rare explicit corrections are not counted."""
+    lanczos_algorithmic_bytes. ``built``: the vectors the device built per attempt (bases are built only
+    as far as a candidate needs); default, the reference's basis sizes. The rare explicit corrections
+    are not counted."""
     total = 0
-    for _, accepted, selected, sizes in schedule:
-        for m in sizes:
+    for i, (_, accepted, selected, sizes) in enumerate(schedule):
+        for m in ([built[i]] if built is not None else sizes):
             total += 16 * n * sum(6 + j for j in range(m))
         if accepted:
             total += 16 * n * (selected + 1) + 8 * n * selected
@@ -185,17 +196,20 @@ def propagate_probe(h, args, horizon=None) -> dict:
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     st = res.stats
-    nbytes = lanczos_algorithmic_bytes(res.schedule, h.dimension)
-    dbytes = lanczos_device_bytes(res.schedule, h.dimension)
+    cgs2_bytes = lanczos_algorithmic_bytes(res.schedule, h.dimension)
+    dbytes = lanczos_device_bytes(res.schedule, h.dimension, prop.device_basis_sizes)
     return {"mode": "propagate", "horizon_us": float(times[-1]), "seconds": dt,
             "simulated_us_per_s": float(times[-1]) / dt, "accepted": st.accepted_steps,
-            "rejected": st.rejected_steps, "matvecs": st.matvecs, "max_krylov": st.maximum_krylov_dimension,
-            "ms_per_matvec_incl_cgs2": dt * 1e3 / st.matvecs,
-            "roofline": {"bound": "hbm", "algorithmic_bytes": nbytes, "achieved": nbytes / dt / 1e9,
-                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": nbytes / dt / 1e9 / HBM_PEAK_GBS,
-                         "definition": "reference CGS2 semantics over the recorded schedule",
-                         "device_bytes": dbytes, "device_achieved": dbytes / dt / 1e9,
-                         "device_frac": dbytes / dt / 1e9 / HBM_PEAK_GBS}}
+            "rejected": st.rejected_steps, "matvecs_reference_count": st.matvecs,
+            "matvecs_device": prop.device_matvecs, "max_krylov": st.maximum_krylov_dimension,
+            "ms_per_device_matvec_incl_orthogonalisation": dt * 1e3 / prop.device_matvecs,
+            # priced on the bytes this implementation must move (one read of the basis per Lanczos step);
+            # the reference's CGS2 would move cgs2_bytes for the same schedule
+            "roofline": {"bound": "hbm", "achieved": dbytes / dt / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": dbytes / dt / 1e9 / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": dbytes,
+                         "definition": "device bytes of the recorded schedule: per Lanczos step j, 16 N (j + 6); "
+                                       "per accepted step the endpoint V c and the singlet projection",
+                         "reference_cgs2_bytes": cgs2_bytes}}
 
 
 # Wall time per full-system trajectory of the reference's archived uncoupled run (n = 232, one
@@ -284,7 +298,7 @@ def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0)
     states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, s) for _, s in cells])
     bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
     res = bp.propagate(states, times, pairs=pairs, progress_callback=progress,
-                       yield_weights=core.yield_weight_table(times, cfg.lifetime_us))
+                       yield_weights=core.yield_weight_table(times, cfg.lifetime_us, core.coarse_grid_indices(cfg)))
     yields = np.array([r.device_yields[0] for r in res])       # summed on the device
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
@@ -296,7 +310,8 @@ def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0)
     return {"mode": "archive_batch", "trajectories": len(cells), "thetas_deg": list(thetas), "samples": samples,
             "quadrupole_scale": quadrupole_scale, "horizon_us": horizon, "seconds": dt,
             "trajectory_us_per_s": len(cells) * horizon / dt, "seconds_per_trajectory": per_traj,
-            "matvecs": int(sum(r.stats.matvecs for r in res)), "rounds": bp.rounds,
+            "matvecs_reference_count": int(sum(r.stats.matvecs for r in res)),
+            "matvecs_device": int(bp.device_matvecs.sum()), "rounds": bp.rounds,
             "max_abs_yield_minus_archived": float(np.abs(diff).max()),
             "max_abs_device_minus_host_yield": float(np.abs(yields - host).max()),
             "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / per_traj}
@@ -396,14 +411,19 @@ def directional_probe(h, args, n_dirs=64, fields_uT=(25.0, 50.0), n_states=2, ho
             "map_mean": float(Y.mean()), "map_range": [float(Y.mean(axis=0).min()), float(Y.mean(axis=0).max())]}
 
 
-def grad_probe(h, args, B=None, horizon=None) -> dict:
-    """Yield-only vs yield + adjoint gradient for a batch of production trajectories over a short horizon."""
+def grad_probe(h, args, B=None, horizon=None, whole=False) -> dict:
+    """Yield-only vs yield + adjoint gradient for a batch of production trajectories; ``whole``: over the
+    whole 57.56 us horizon (46,053-point grid, production budget), else over a short horizon with the
+    horizon-scaled budget."""
     import numpy as np
     import torch
     from rpm_amd import adjoint, batch, core
     cfg = core.SweepConfig(lifetime_us=5.0, survival_tail_bound=1e-5)
-    horizon_req = args.horizon if horizon is None else horizon
-    times = np.arange(int(round(horizon_req / 0.00125)) + 1) * 0.00125
+    if whole:
+        times = cfg.time_grid_us
+    else:
+        horizon_req = args.horizon if horizon is None else horizon
+        times = np.arange(int(round(horizon_req / 0.00125)) + 1) * 0.00125
     B = args.batch if B is None else B
     thetas = list(np.linspace(10.0, 170.0, B))
     phis = [0.0] * B
@@ -503,11 +523,13 @@ def pulse_probe(vectors: int = 8, segments: int = 64, window_us: float = 2.0, ma
     gmax = float(np.abs(g).max())
     step = max_move_uT / gmax if gmax > 0.0 else 0.0
     y1 = prob.forward(*fixed, np.tile(amps + step * g, (vectors, 1)))
+    predicted = step * float(np.dot(g, g))      # first-order change of the mean yield along the step
     return {"mode": "pulse", "vectors": vectors, "segments": segments, "window_us": window_us,
             "horizon_us": float(times[-1]), "seconds_forward": t_forward, "seconds_sweep": t_sweep,
             "seconds_per_vector_step": (t_forward + t_sweep) / vectors,
             "yield_plus_grad_over_yield": (t_forward + t_sweep) / t_forward,
             "mean_yield": float(y0.mean()), "mean_yield_after_step": float(y1.mean()),
+            "observed_change": float(y1.mean() - y0.mean()), "predicted_change_first_order": predicted,
             "gradient_max_abs_per_uT": float(np.abs(g).max()), "psi0_recovery_error": rec}
 
 
@@ -770,24 +792,31 @@ def main() -> int:
         result["cpu_baseline"] = cpu_baseline()
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
     if not args.no_extras:
-        # the other two parts of the metric, from short probes of the production operator
+        # the other two parts of the metric over the whole 57.56 us horizon (the reference's unit of work
+        # is a whole trajectory): batched yields/s against the plain one-trajectory path, and the
+        # yield + gradient time over the yield time
         h = core.build_hamiltonian(90.0, 0.05)
         del x
         torch.cuda.empty_cache()
-        print("[bench] batched-trajectory probe", file=sys.stderr, flush=True)
-        result["batched"] = batch_probe(h, args, B=32, horizon=0.25, plain_sample=4)
+        print("[bench] whole-trajectory probe (plain path)", file=sys.stderr, flush=True)
+        result["trajectory"] = trajectory_probe()
+        print("[bench] 32 whole archived trajectories batched", file=sys.stderr, flush=True)
+        ab = archive_batch_probe()
+        plain = result["trajectory"]["seconds"]
+        ab.update({"yields_per_s": ab["trajectories"] / ab["seconds"], "plain_seconds_per_trajectory": plain,
+                   "yields_per_s_plain": 1.0 / plain, "speedup_vs_plain": plain / ab["seconds_per_trajectory"],
+                   "plain_cell": result["trajectory"]["cell"]})
+        result["batched"] = ab
+        print("[bench] whole-horizon gradient probe", file=sys.stderr, flush=True)
+        result["gradient"] = grad_probe(h, args, B=8, whole=True)
         print("[bench] directional-map probe", file=sys.stderr, flush=True)
         result["directional"] = directional_probe(h, args)
-        print("[bench] gradient probe", file=sys.stderr, flush=True)
-        result["gradient"] = grad_probe(h, args, B=8, horizon=0.05)
         print("[bench] factorized spectral probe", file=sys.stderr, flush=True)
         result["spectral"] = spectral_probe(args, with_cpu=not args.no_cpu)
         print("[bench] Sobol reaction-time probe", file=sys.stderr, flush=True)
         result["sobol"] = sobol_probe(args)
         print("[bench] propagation probe", file=sys.stderr, flush=True)
         result["propagation"] = propagate_probe(h, args, horizon=0.1)
-        print("[bench] full-trajectory probe", file=sys.stderr, flush=True)
-        result["trajectory"] = trajectory_probe()
     if not args.no_cpu and not args.no_extras:
         print("[bench] CPU trajectory and all-core baselines", file=sys.stderr, flush=True)
         extra = cpu_extra_baselines()

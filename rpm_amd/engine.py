@@ -669,9 +669,10 @@ class AdaptiveLanczosPropagator:
         self.breakdown_tolerance = breakdown_tolerance
         self._ws = None
         self._built = 0
-        # matvecs the device made in the last propagate call; stats.matvecs (and the Hamiltonian's
-        # matvec_count) count the reference's whole-basis calls for the same schedule
+        # matvecs the device made in the last propagate call, and the vectors of each attempt's basis;
+        # stats.matvecs (and the Hamiltonian's matvec_count) count the reference's whole-basis calls
         self.device_matvecs = 0
+        self.device_basis_sizes = []
 
     # -- device pieces ------------------------------------------------
 
@@ -838,6 +839,7 @@ class AdaptiveLanczosPropagator:
         prefer_max = False
         guess = self.maximum_dimension   # basis prefix to build first in prefer-max mode
         self.device_matvecs = 0
+        self.device_basis_sizes = []     # vectors the device built per attempt (in schedule order)
         ws = None
         dev = nat.device()
         new_state = torch.empty_like(state)
@@ -897,6 +899,7 @@ class AdaptiveLanczosPropagator:
                 sizes = [ref]
                 exact = ref < mx
             h.matvec_count += sum(sizes)      # the reference's count for this schedule
+            self.device_basis_sizes.append(self._built)
             rejected_now = selected is None or (passes is None and err > budget)
             if rejected_now:
                 rejected += 1
