@@ -152,6 +152,9 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
   }
 }
 
+// h[c] = sum_blk partial[c][blk]; optionally alpha[j] = Re h[alpha_col]. With dpart, the last block
+// of each trajectory instead forms dsum[b] = sum_blk dpart[b][blk] (the |y|^2 of the lagged pass),
+// in the order k_lcgs_finish used to.
 __global__ void __launch_bounds__(kThreads) k_reduce_cols(const c128* __restrict__ partial, int nblk, long long pstride,
                                                           c128* __restrict__ h, int hstride, double* __restrict__ alpha,
                                                           int astride, int alpha_col, int j, int need,
@@ -224,31 +227,132 @@ constexpr double kFixThreshold = 1e-6;     // |c_{j+1}| above which the correcti
 constexpr double kCancel = 1e-2;           // beta^2 <= kCancel |y|^2: the norm is taken explicitly
 constexpr int kFixBlocks = 256;            // blocks per trajectory of k_lcgs_finish
 constexpr int kPassWaves = 4;
-// columns per wave and round, rounds: for j <= 16, j <= 32 and j <= 64. Measured (32 trajectories,
-// batched trajectory-us/s): one round of 8 or 16 columns 20.8, rounds of 8 22.3, of 4 22.9, of 2
-// 23.1; for j <= 16, rounds of 2 gave 23.15 against 23.0 for one round of 4.
-constexpr int kRound16[2] = {2, 2};
-constexpr int kRound32[2] = {2, 4};
-constexpr int kRound64[2] = {2, 8};
 static_assert(kPassWaves * kWave == kThreads, "k_lcgs_pass shares the kThreads-wide reductions");
 
-// Columns c < j are dealt to the waves: column c to wave c % 4, CPR per wave and round, ROUNDS
-// rounds (columns q + 4 (round CPR + i)). Whatever the split, a column's partial is formed by the
-// same lanes over the same elements in the same order, and each element's correction adds the
-// wave's columns in ascending order (carried in LDS from one round to the next) and then the waves
-// in a fixed order, so the basis is bitwise the same for every split and batch size. Rounds keep
-// the accumulators of few columns in registers at a time (higher occupancy). kStream: the basis
-// columns are read with non-temporal loads.
-// That pays when the bases of a batch are far larger than the 256 MB Infinity Cache (+5% at 32
-// trajectories) and costs when one trajectory's basis fits in it (-10%).
-template <int CPR, int ROUNDS, bool kStream>
+// One block per kPassElems elements of one trajectory's vectors; columns c < j are dealt to the waves,
+// column c to wave c % 4, and each wave streams its columns one at a time with the next column's
+// loads in flight (a round structure that waited for every load of a round before reducing it was
+// latency-bound: 977 us per launch whatever j). kStream: the basis columns are read with
+// non-temporal loads. That pays when the bases of a batch are far larger than the 256 MB Infinity
+// Cache (+5% at 32 trajectories) and costs when one trajectory's basis fits in it (-10%).
+// The four loads of one column for this lane: elements it * kWave + lane of the block (kFull: all in
+// range, so every load is a constant offset from the column start; else clamped to the last element,
+// which then weighs nothing).
+template <bool kStream, bool kFull>
+__device__ inline void pass_load(const c128* __restrict__ col, int lane, int cnt, c128 (&v)[kPassPer]) {
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) {
+    const int e = kFull ? it * kWave + lane : min(it * kWave + lane, cnt - 1);
+    if constexpr (kStream) {
+      v[it].re = __builtin_nontemporal_load(&col[e].re);
+      v[it].im = __builtin_nontemporal_load(&col[e].im);
+    } else {
+      v[it] = col[e];
+    }
+  }
+}
+
+// Body of k_lcgs_pass for one block (kThreads == kPassElems: thread t owns element x0 + t).
+template <bool kStream, bool kFull>
+__device__ inline void pass_body(c128* __restrict__ Vb, long long n, int j, long long x0, int cnt,
+                                 const c128* __restrict__ xin, c128* __restrict__ u, const c128* __restrict__ cpend_b,
+                                 double at, double bprev, double sc, c128* __restrict__ part_b,
+                                 double* __restrict__ ypart_b, c128* s_u, c128 (*s_corr)[kPassElems], c128* s_c,
+                                 c128* sm, double* smd, int blk, int nblk) {
+  const int t = threadIdx.x, lane = t % kWave, q = t / kWave;
+  const bool mine = t < cnt;
+  const int et = mine ? t : cnt - 1;
+  // This wave's columns q, q + 4, q + 8, ... < j are streamed one at a time with the next columns'
+  // loads in flight. Each column's partial sums the wave's elements in the order it = 0, 1, ... and
+  // the correction of each element adds the wave's columns in ascending order, then the waves in a
+  // fixed order, so the basis is bitwise the same for every batch size.
+  const int ncol = j > q ? (j - q + kPassWaves - 1) / kPassWaves : 0;
+  c128 va[kPassPer], vb[kPassPer];
+  // the wave's first column is requested before the block stages u (which it does not need)
+  if (ncol > 0) pass_load<kStream, kFull>(Vb + static_cast<long long>(q) * n + x0, lane, cnt, va);
+  {
+    c128 ue = u[x0 + et];                       // u = H x_j - beta_{j-1} v_{j-1} (three-term step)
+    if (j > 0) {
+      const c128 p = Vb[static_cast<long long>(j - 1) * n + x0 + et];
+      ue.re = fma(-bprev, p.re, ue.re);
+      ue.im = fma(-bprev, p.im, ue.im);
+    }
+    s_u[t] = mine ? ue : cmake(0.0, 0.0);       // elements past cnt weigh nothing
+  }
+  if (t < j) s_c[t] = cpend_b[t];
+  __syncthreads();
+  c128 corr[kPassPer];
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) corr[it] = cmake(0.0, 0.0);
+  auto load = [&](int k, c128(&v)[kPassPer]) {
+    pass_load<kStream, kFull>(Vb + static_cast<long long>(q + kPassWaves * k) * n + x0, lane, cnt, v);
+  };
+  auto use = [&](int k, const c128(&v)[kPassPer]) {
+    const int c = q + kPassWaves * k;
+    const c128 cc = s_c[c];
+    c128 acc = cmake(0.0, 0.0);
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) {
+      acc = cfmaconj(v[it], s_u[it * kWave + lane], acc);
+      corr[it] = cfma(v[it], cc, corr[it]);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) part_b[static_cast<long long>(c) * nblk + blk] = acc;
+  };
+  // the loop body has no branch, so the next column's loads stay outstanding until their use (a load
+  // under a branch would make the compiler wait for everything at the join); measured: a third
+  // column in flight, or fewer registers for more waves, does not change the time per column
+  int k = 0;
+  for (; k + 2 < ncol; k += 2) {
+    load(k + 1, vb);
+    use(k, va);
+    load(k + 2, va);
+    use(k + 1, vb);
+  }
+  if (k + 1 < ncol) {
+    load(k + 1, vb);
+    use(k, va);
+    use(k + 1, vb);
+  } else if (k < ncol) {
+    use(k, va);
+  }
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) s_corr[q][it * kWave + lane] = corr[it];
+  __syncthreads();
+  // v_j = s x_j - V_<j c_j, y = u - alpha~ v_j, gamma = v_j^H u, |y|^2
+  c128 g = cmake(0.0, 0.0);
+  double yy = 0.0;
+  if (mine) {
+    const c128 ue = s_u[t];
+    const c128 cs = cadd(cadd(cadd(s_corr[0][t], s_corr[1][t]), s_corr[2][t]), s_corr[3][t]);
+    const c128 v = csub(cscale(xin[x0 + t], sc), cs);
+    Vb[static_cast<long long>(j) * n + x0 + t] = v;
+    const c128 y = cmake(fma(-at, v.re, ue.re), fma(-at, v.im, ue.im));
+    u[x0 + t] = y;
+    g = cfmaconj(v, ue, g);
+    yy = fma(y.re, y.re, fma(y.im, y.im, yy));
+  }
+  g = block_sum(g, sm);
+  yy = block_sum(yy, smd);
+  if (t == 0) {
+    part_b[static_cast<long long>(j) * nblk + blk] = g;
+    ypart_b[blk] = yy;
+  }
+}
+
+// Grid (blocks, batch): block blockIdx.x is element block blk0 + blockIdx.x of nblk. kFull: every
+// element of those blocks is in range (all but a vector's last block when kPassElems does not divide
+// N, which a second launch with kFull = false covers).
+template <bool kStream, bool kFull>
 __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, long long vstride, long long n, int j,
                                                         const c128* __restrict__ in, long long ild,
                                                         const c128* __restrict__ cpend, int cstride,
                                                         double* __restrict__ atilde,
                                                         const c128* __restrict__ adots, int units, c128* __restrict__ U,
                                                         c128* __restrict__ partial, long long pstride,
-                                                        double* __restrict__ ypart, const BasisCtl* ctl) {
+                                                        double* __restrict__ ypart, const BasisCtl* ctl, int blk0,
+                                                        int nblk) {
+  static_assert(kPassElems == kThreads, "one element per thread outside the column stream");
   const int b = blockIdx.y;
   if (!basis_live(ctl, b, j, 0)) return;
   __shared__ c128 s_u[kPassElems];
@@ -261,90 +365,17 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
     // alpha~ from the matvec's partials, formed by every block in the same order as k_alpha;
     // block 0 stores it for k_lcgs_finish
     at = alpha_sum(adots, units, b, sm);
-    if (blockIdx.x == 0 && threadIdx.x == 0) atilde[b] = at;
+    if (blk0 + blockIdx.x == 0 && threadIdx.x == 0) atilde[b] = at;
   } else {
     at = atilde[b];
   }
-  c128* Vb = V + b * vstride;
-  c128* u = U + b * n;
-  const long long x0 = static_cast<long long>(blockIdx.x) * kPassElems;
-  const int cnt = static_cast<int>(n - x0 < kPassElems ? n - x0 : kPassElems);
-  // u = H x_j - beta_{j-1} v_{j-1}, the three-term step's beta term
-  const double bprev = ctl[b].beta_prev;
-  const c128* vprev = j > 0 ? Vb + static_cast<long long>(j - 1) * n : nullptr;
-  for (int e = threadIdx.x; e < cnt; e += kThreads) {
-    c128 uu = u[x0 + e];
-    if (vprev) {
-      const c128 p = vprev[x0 + e];
-      uu.re = fma(-bprev, p.re, uu.re);
-      uu.im = fma(-bprev, p.im, uu.im);
-    }
-    s_u[e] = uu;
-  }
-  if (threadIdx.x < j) s_c[threadIdx.x] = cpend[b * cstride + threadIdx.x];
-  __syncthreads();
-  const int lane = threadIdx.x % kWave, q = threadIdx.x / kWave;
-  const int nblk = gridDim.x;
-#pragma unroll 1
-  for (int rd = 0; rd < ROUNDS; ++rd) {
-    c128 acc[CPR];
-#pragma unroll
-    for (int i = 0; i < CPR; ++i) acc[i] = cmake(0.0, 0.0);
-    for (int it = 0; it < kPassPer; ++it) {
-      const int e = it * kWave + lane;
-      c128 corr = rd == 0 ? cmake(0.0, 0.0) : s_corr[q][e];   // this lane's own running sum
-      if (e < cnt) {
-        const c128 uu = s_u[e];
-#pragma unroll
-        for (int i = 0; i < CPR; ++i) {
-          const int c = q + kPassWaves * (rd * CPR + i);
-          if (c < j) {
-            const c128* pv = Vb + static_cast<long long>(c) * n + x0 + e;
-            c128 v;
-            if constexpr (kStream) {
-              v.re = __builtin_nontemporal_load(&pv->re);
-              v.im = __builtin_nontemporal_load(&pv->im);
-            } else {
-              v = *pv;
-            }
-            acc[i] = cfmaconj(v, uu, acc[i]);
-            corr = cfma(v, s_c[c], corr);
-          }
-        }
-      }
-      s_corr[q][e] = corr;
-    }
-#pragma unroll
-    for (int i = 0; i < CPR; ++i) {
-      const int c = q + kPassWaves * (rd * CPR + i);
-      if (c < j) {
-        const c128 t = wave_sum(acc[i]);
-        if (lane == 0) partial[b * pstride + static_cast<long long>(c) * nblk + blockIdx.x] = t;
-      }
-    }
-  }
-  __syncthreads();
-  c128* vj = Vb + static_cast<long long>(j) * n;
-  const c128* xin = in + b * ild;
-  const double sc = ctl[b].scale;
-  c128 g = cmake(0.0, 0.0);
-  double yy = 0.0;
-  for (int e = threadIdx.x; e < cnt; e += kThreads) {
-    const c128 corr = cadd(cadd(cadd(s_corr[0][e], s_corr[1][e]), s_corr[2][e]), s_corr[3][e]);
-    const c128 v = csub(cscale(xin[x0 + e], sc), corr);
-    vj[x0 + e] = v;
-    const c128 uu = s_u[e];
-    const c128 y = cmake(fma(-at, v.re, uu.re), fma(-at, v.im, uu.im));
-    u[x0 + e] = y;
-    g = cfmaconj(v, uu, g);
-    yy = fma(y.re, y.re, fma(y.im, y.im, yy));
-  }
-  g = block_sum(g, sm);
-  yy = block_sum(yy, smd);
-  if (threadIdx.x == 0) {
-    partial[b * pstride + static_cast<long long>(j) * nblk + blockIdx.x] = g;
-    ypart[static_cast<long long>(b) * nblk + blockIdx.x] = yy;
-  }
+  const int blk = blk0 + blockIdx.x;
+  const long long x0 = static_cast<long long>(blk) * kPassElems;
+  const int cnt = kFull ? kPassElems : static_cast<int>(n - x0 < kPassElems ? n - x0 : kPassElems);
+  const BasisCtl& C = ctl[b];
+  pass_body<kStream, kFull>(V + b * vstride, n, j, x0, cnt, in + b * ild, U + b * n, cpend + b * cstride, at,
+                            C.beta_prev, C.scale, partial + b * pstride, ypart + static_cast<long long>(b) * nblk,
+                            s_u, s_corr, s_c, sm, smd, blk, nblk);
 }
 
 // alpha_j, beta_j and c_{j+1} from the pass's sums h = [D; gamma] and |y|^2 (see above), and the
@@ -678,17 +709,27 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   return w;
 }
 
-template <int CPR, int ROUNDS>
+template <bool kStream>
+void launch_pass_kind(const BasisWs& w, c128* V, long long vstride, long long n, int j, const c128* in, long long ild,
+                      int m, bool fuse_alpha, c128* out, int batch, hipStream_t s) {
+  const int nfull = static_cast<int>(n / kPassElems);
+  const c128* adots = fuse_alpha ? w.dots : nullptr;
+  if (nfull > 0)
+    hipLaunchKernelGGL((k_lcgs_pass<kStream, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j, in,
+                       ild, w.cpend, m, w.atilde, adots, w.units, out, w.partial, w.pstride, w.ypart, w.ctl, 0,
+                       w.npass);
+  if (nfull < w.npass)
+    hipLaunchKernelGGL((k_lcgs_pass<kStream, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
+                       n, j, in, ild, w.cpend, m, w.atilde, adots, w.units, out, w.partial, w.pstride, w.ypart, w.ctl,
+                       nfull, w.npass);
+}
+
 void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int j, const c128* in, long long ild, int m,
                  bool fuse_alpha, c128* out, int batch, hipStream_t s) {
   if (batch > 1)
-    hipLaunchKernelGGL((k_lcgs_pass<CPR, ROUNDS, true>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                       in, ild, w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
-                       w.pstride, w.ypart, w.ctl);
+    launch_pass_kind<true>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
   else
-    hipLaunchKernelGGL((k_lcgs_pass<CPR, ROUNDS, false>), dim3(w.npass, batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                       in, ild, w.cpend, m, w.atilde, fuse_alpha ? w.dots : nullptr, w.units, out, w.partial,
-                       w.pstride, w.ypart, w.ctl);
+    launch_pass_kind<false>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
 }
 
 // Shared implementation of the single and batched basis builders.
@@ -737,12 +778,7 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
       RPM_LAUNCH_CHECK();
     }
     // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
-    if (j <= kPassWaves * 4)
-      launch_pass<kRound16[0], kRound16[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
-    else if (j <= kPassWaves * 8)
-      launch_pass<kRound32[0], kRound32[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
-    else
-      launch_pass<kRound64[0], kRound64[1]>(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
+    launch_pass(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_reduce_cols, dim3(j + 2, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
                        nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
