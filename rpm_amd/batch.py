@@ -32,7 +32,7 @@ import numpy as np
 
 from . import _native as nat
 from .engine import (
-    CompiledFullPairHamiltonian, PropagationResult, PropagationStats, YieldAccumulator, _pair_form,
+    BasisGraphs, CompiledFullPairHamiltonian, PropagationResult, PropagationStats, YieldAccumulator, _pair_form,
     state_error_to_population_bound,
 )
 
@@ -130,6 +130,9 @@ class BatchedLanczosPropagator:
         # matvecs the device made per trajectory in the last call; stats.matvecs counts the reference's
         # whole-basis calls for the same schedule (bases are built only as far as a candidate needs)
         self.device_matvecs = None
+        # replay each basis step window as a HIP graph (captured once per propagate call)
+        self.use_graphs = False
+        self.graphs = None
 
     @classmethod
     def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
@@ -368,6 +371,10 @@ class BatchedLanczosPropagator:
         cand_err = torch.zeros(B * ladder_len, dtype=torch.float64, device=dev)
         size_h = np.zeros(B, dtype=np.int64)
         red_ws = nat.workspace(lib.rpm_reduce_workspace_bytes(n, B))
+        d_ts = torch.zeros(2 * B, dtype=torch.int32, device=dev)     # basis targets and stops of a call
+        # each basis step window as a HIP graph, captured once for this call's buffers and replayed
+        graphs = BasisGraphs() if self.use_graphs else None
+        self.graphs = graphs
         nrm_out = torch.empty(B, dtype=torch.float64, device=dev)
         pop_out = torch.empty(B, dtype=torch.float64, device=dev)
         sp = ctypes.c_void_p(nat.stream_ptr())
@@ -421,7 +428,7 @@ class BatchedLanczosPropagator:
                     for b in group:
                         resume(b, (abs(float(nv[b]) - 1.0), float(pv[b])))
                 elif phase == _BASIS:
-                    # targets (0: continue an earlier prefix) then stops, one upload
+                    # targets (0: continue an earlier prefix) then stops, into the call's own buffer
                     ts = np.zeros(2 * B, dtype=np.int32)
                     j0 = m
                     for b in group:
@@ -432,12 +439,20 @@ class BatchedLanczosPropagator:
                         else:
                             ts[b] = min(int(target), n)
                             j0 = 0
-                    d_ts = torch.from_numpy(ts).to(dev)
-                    nat.check(lib.rpm_lanczos_basis_batch_prefix(
-                        vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_ts)), m, vp(nat.ptr(d_ts) + 4 * B),
-                        j0, int(ts[B:].max()), vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)),
-                        m, vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
-                        float(self.breakdown_tolerance), vp(nat.ptr(ws)), ws.numel(), sp))
+                    d_ts.copy_(torch.from_numpy(ts))
+                    max_stop = int(ts[B:].max())
+
+                    def launch(stream, j0=j0, max_stop=max_stop):
+                        nat.check(lib.rpm_lanczos_basis_batch_prefix(
+                            vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_ts)), m, vp(nat.ptr(d_ts) + 4 * B),
+                            j0, max_stop, vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)), m,
+                            vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
+                            float(self.breakdown_tolerance), vp(nat.ptr(ws)), ws.numel(), vp(stream)))
+
+                    if graphs is not None:
+                        graphs.run((j0, max_stop), launch)
+                    else:
+                        launch(nat.stream_ptr())
                     inf_h = info.cpu().numpy()
                     nrm_h = norm.cpu().numpy()
                     for b in group:

@@ -27,7 +27,9 @@
 // (LDS atomics) varies.
 #include <rocsolver/rocsolver.h>
 
+#include <memory>
 #include <mutex>
+#include <vector>
 
 #include "rpm_common.hpp"
 
@@ -355,13 +357,34 @@ extern "C" int rpm_spectral_yields(const void* d_fa, const void* d_fb, const voi
 namespace {
 
 constexpr int kMaxDevices = 64;
-std::mutex g_solver_mutex;
-rocblas_handle g_solver[kMaxDevices] = {};
 
-rocblas_handle solver_handle(int dev) {
+// One rocBLAS handle per (device, stream), bound to its stream when created: a handle's internal
+// device workspace is then only used by kernels of that one stream, which run in order. Its mutex is
+// held for each whole solve, since a handle must not be used by two host threads at once. Handles
+// live until the process ends.
+struct SolverSlot {
+  int dev;
+  hipStream_t stream;
+  rocblas_handle handle;
+  std::mutex lock;
+};
+std::mutex g_solver_mutex;
+std::vector<std::unique_ptr<SolverSlot>> g_solvers;
+
+SolverSlot* solver_slot(int dev, hipStream_t stream) {
   std::lock_guard<std::mutex> lock(g_solver_mutex);
-  if (!g_solver[dev] && rocblas_create_handle(&g_solver[dev]) != rocblas_status_success) g_solver[dev] = nullptr;
-  return g_solver[dev];
+  for (auto& s : g_solvers)
+    if (s->dev == dev && s->stream == stream) return s.get();
+  auto slot = std::make_unique<SolverSlot>();
+  slot->dev = dev;
+  slot->stream = stream;
+  if (rocblas_create_handle(&slot->handle) != rocblas_status_success) return nullptr;
+  if (rocblas_set_stream(slot->handle, stream) != rocblas_status_success) {
+    rocblas_destroy_handle(slot->handle);
+    return nullptr;
+  }
+  g_solvers.push_back(std::move(slot));
+  return g_solvers.back().get();
 }
 
 size_t align256(size_t n) { return (n + 255) / 256 * 256; }
@@ -381,10 +404,10 @@ extern "C" int rpm_eigh_batched(void* d_a, int n, int batch, double* d_w, void* 
   int dev = 0;
   RPM_HIP_CHECK(hipGetDevice(&dev));
   if (dev < 0 || dev >= kMaxDevices) return fail(RPM_E_UNSUPPORTED, "rpm_eigh_batched: device index out of range");
-  rocblas_handle h = solver_handle(dev);
-  if (!h) return fail(RPM_E_HIP, "rpm_eigh_batched: rocblas_create_handle failed");
-  if (rocblas_set_stream(h, static_cast<hipStream_t>(stream)) != rocblas_status_success)
-    return fail(RPM_E_HIP, "rpm_eigh_batched: rocblas_set_stream failed");
+  SolverSlot* slot = solver_slot(dev, static_cast<hipStream_t>(stream));
+  if (!slot) return fail(RPM_E_HIP, "rpm_eigh_batched: rocBLAS handle for this stream could not be created");
+  std::lock_guard<std::mutex> hold(slot->lock);
+  rocblas_handle h = slot->handle;
   auto* info = static_cast<rocblas_int*>(d_ws);
   auto* e = reinterpret_cast<double*>(static_cast<char*>(d_ws) + align256(sizeof(int) * static_cast<size_t>(batch)));
   auto* a = static_cast<rocblas_double_complex*>(d_a);

@@ -458,6 +458,39 @@ def device_norm(x, out=None):
     return out
 
 
+class BasisGraphs:
+    """HIP graphs of the basis builder's launch sequences, one per (first step, last step) window.
+
+    The builder launches kernels only (no allocation, host synchronisation or memset), so each window
+    is captured once (on a side stream, torch.cuda.graph) and replayed on the current stream; the
+    per-call targets and stops are read from device memory at replay. A cache belongs to one set of
+    buffers: build a new one when the buffers change.
+    """
+
+    def __init__(self, owner=None):
+        self.owner = owner          # the buffers the graphs were captured with
+        self.graphs = {}
+        self.captures = 0
+        self.replays = 0
+
+    def run(self, key, launch):
+        """``launch(stream_handle)`` issues the window's kernels on the given stream."""
+        torch = nat.require_gpu()
+        g = self.graphs.get(key)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            cur = torch.cuda.current_stream()
+            side = torch.cuda.Stream()
+            side.wait_stream(cur)
+            with torch.cuda.graph(g, stream=side):
+                launch(torch.cuda.current_stream().cuda_stream)
+            cur.wait_stream(side)
+            self.graphs[key] = g
+            self.captures += 1
+        g.replay()
+        self.replays += 1
+
+
 class YieldAccumulator:
     """Device weight tables and (Phi_S, coarse Phi_S, estimate) accumulators of B trajectories.
 
@@ -637,6 +670,7 @@ class _KrylovWorkspace:
         self.scratch = nat.workspace(lib.rpm_krylov_scratch_bytes(16, m))
         self.gram_ws = nat.workspace(lib.rpm_singlet_gram_workspace_bytes(m, op.n // 4))
         self.P = torch.zeros((m, m), dtype=torch.complex128, device=dev)
+        self.targets = torch.zeros(2, dtype=torch.int32, device=dev)   # basis target and stop of the next call
 
 
 class AdaptiveLanczosPropagator:
@@ -669,6 +703,9 @@ class AdaptiveLanczosPropagator:
         self.breakdown_tolerance = breakdown_tolerance
         self._ws = None
         self._built = 0
+        # replay each basis step window as a HIP graph (captured once per propagation)
+        self.use_graphs = False
+        self._graphs = None
         # matvecs the device made in the last propagate call, and the vectors of each attempt's basis;
         # stats.matvecs (and the Hamiltonian's matvec_count) count the reference's whole-basis calls
         self.device_matvecs = 0
@@ -687,7 +724,8 @@ class AdaptiveLanczosPropagator:
     def _basis(self, state, target, stop=None, resume=False):
         """Device Lanczos basis in prefixes: a new basis of eventual size ``target`` built to ``stop``
         vectors (all of them by default), or (``resume``) the current one continued to ``stop``.
-        Returns the vectors built."""
+        Returns the vectors built. With ``use_graphs`` the launch sequence of each step window is a
+        replayed HIP graph (its state buffer is then the propagation's own copy)."""
         if stop is None:
             stop = target
         torch = nat.require_gpu()
@@ -697,14 +735,23 @@ class AdaptiveLanczosPropagator:
         lib = nat.load()
         cap = min(ws.m, n)
         ts = np.array([0 if resume else min(int(target), cap), min(int(stop), cap)], dtype=np.int32)
-        d_ts = torch.from_numpy(ts).to(state.device)
+        ws.targets.copy_(torch.from_numpy(ts))
         j0 = self._built if resume else 0
-        nat.check(lib.rpm_lanczos_basis_batch_prefix(
-            ctypes.c_void_p(op.handle), 1, ctypes.c_void_p(nat.ptr(state)), n, ctypes.c_void_p(nat.ptr(d_ts)), ws.m,
-            ctypes.c_void_p(nat.ptr(d_ts) + 4), j0, int(ts[1]), None, ctypes.c_void_p(nat.ptr(ws.V)), ws.m,
-            ctypes.c_void_p(nat.ptr(ws.alpha)), ctypes.c_void_p(nat.ptr(ws.beta)), ctypes.c_void_p(nat.ptr(ws.info)),
-            ctypes.c_void_p(nat.ptr(ws.norm)), float(self.breakdown_tolerance), ctypes.c_void_p(nat.ptr(ws.ws)),
-            ws.ws.numel(), ctypes.c_void_p(nat.stream_ptr())))
+        graphs = self._graphs if (self._graphs is not None and self._graphs.owner is ws) else None
+
+        def launch(stream):
+            nat.check(lib.rpm_lanczos_basis_batch_prefix(
+                ctypes.c_void_p(op.handle), 1, ctypes.c_void_p(nat.ptr(state)), n,
+                ctypes.c_void_p(nat.ptr(ws.targets)), ws.m, ctypes.c_void_p(nat.ptr(ws.targets) + 4), j0, int(ts[1]),
+                None, ctypes.c_void_p(nat.ptr(ws.V)), ws.m, ctypes.c_void_p(nat.ptr(ws.alpha)),
+                ctypes.c_void_p(nat.ptr(ws.beta)), ctypes.c_void_p(nat.ptr(ws.info)), ctypes.c_void_p(nat.ptr(ws.norm)),
+                float(self.breakdown_tolerance), ctypes.c_void_p(nat.ptr(ws.ws)), ws.ws.numel(),
+                ctypes.c_void_p(stream)))
+
+        if graphs is not None:
+            graphs.run((nat.ptr(state), j0, int(ts[1])), launch)
+        else:
+            launch(nat.stream_ptr())
         size = int(ws.info[0].item())
         self.device_matvecs += size - j0
         self._built = size
@@ -838,6 +885,7 @@ class AdaptiveLanczosPropagator:
         step = min(self.initial_step_us, t_final) if t_final else 0.0
         prefer_max = False
         guess = self.maximum_dimension   # basis prefix to build first in prefer-max mode
+        self._graphs = BasisGraphs(self._workspace()) if self.use_graphs else None
         self.device_matvecs = 0
         self.device_basis_sizes = []     # vectors the device built per attempt (in schedule order)
         ws = None

@@ -156,3 +156,41 @@ def test_multiplet_blocks_match_the_whole_radical(scale):
         assert a.keys() == b.keys()
         for k in a:
             assert abs(a[k] - b[k]) < 1e-12, (a["theta_deg"], k, a[k], b[k])
+
+
+def test_eigh_from_two_host_threads_on_two_streams():
+    """rpm_eigh_batched keeps one rocBLAS handle per stream and holds it for the whole solve, so host
+    threads on different streams get the same eigensystems as serial calls."""
+    import threading
+    print("case: eigh from two threads, two streams", flush=True)
+    rng = np.random.default_rng(3)
+    mats = []
+    for _ in range(2):
+        a = rng.normal(size=(8, 96, 96)) + 1j * rng.normal(size=(8, 96, 96))
+        mats.append(np.ascontiguousarray(a + np.conj(np.transpose(a, (0, 2, 1)))))
+    ref = [ps.eigh_batched(torch.from_numpy(m.copy()).cuda())[0].cpu().numpy() for m in mats]
+    out, errors = [None, None], []
+
+    def work(i):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                for _ in range(6):
+                    w, vh = ps.eigh_batched(torch.from_numpy(mats[i].copy()).to("cuda", non_blocking=False))
+                s.synchronize()
+                out[i] = (w.cpu().numpy(), vh.cpu().numpy())
+        except Exception as exc:   # reported after the join
+            errors.append(exc)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not errors, errors
+    for i in range(2):
+        w, vh = out[i]
+        np.testing.assert_allclose(w, ref[i], rtol=0, atol=1e-11)
+        v = np.conj(np.transpose(vh, (0, 2, 1)))
+        rec = v @ (w[:, :, None] * vh)
+        assert np.abs(rec - mats[i]).max() < 1e-10
