@@ -23,6 +23,7 @@ reference's 0.1 us speed probe, priced on the bytes this implementation moves).
     python bench.py --mode archive_batch   # 32 whole archived trajectories in one batch (about a minute)
     python bench.py --mode gradcheck       # whole-horizon adjoint gradient against central differences
     python bench.py --mode pulse           # one whole-horizon ascent step of the 64-segment pulse
+    python bench.py --mode ceiling         # copy and two-pass streaming bounds beside the H·psi kernel
 """
 
 from __future__ import annotations
@@ -626,6 +627,53 @@ def spectral_probe(args, with_cpu=True) -> dict:
     return out
 
 
+def ceiling_probe(h, B: int = 256, reps: int = 20) -> dict:
+    """The streaming bounds an H.psi design over B resident vectors cannot beat, measured on this box
+    with plain device copies (no stencil work), next to the fused kernel itself:
+    - "copy": read B vectors and write B vectors once (the bytes of a single fused pass);
+    - "two_pass_chunked": per chunk of c vectors, a copy (the first pass writing an intermediate) and
+      an add-to (the second pass reading the input and the intermediate, writing the result), with
+      chunks small enough (c <= 32: at most 113 MB per array) for the 256 MB Infinity Cache to hold
+      the intermediate: the bound of a design that splits H into a FAD pass and a TrpH pass;
+    - "kernel": the H.psi kernel at the same batch.
+    Times per B vectors from HIP events on the current stream."""
+    import torch
+    from rpm_amd import workloads
+    n = h.dimension
+    x = torch.from_numpy(workloads.normal_vectors(n, B, seed=0)).cuda()
+    y = torch.empty_like(x)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    out = {"mode": "ceiling", "batch": B, "dimension": n, "bytes_per_vector": BYTES_PER_AMPLITUDE * n}
+    algo = BYTES_PER_AMPLITUDE * n * B
+    ms = timed(lambda: y.copy_(x))
+    out["copy"] = {"ms": ms, "GBps": algo / (ms / 1e3) / 1e9, "frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    chunks = {}
+    for c in (8, 16, 32):
+        def two_pass(c=c):
+            for i in range(0, B, c):
+                y[i:i + c].copy_(x[i:i + c])
+                y[i:i + c].add_(x[i:i + c])
+        ms = timed(two_pass)
+        chunks[str(c)] = {"ms": ms, "GBps_algorithmic": algo / (ms / 1e3) / 1e9,
+                          "frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    out["two_pass_chunked"] = chunks
+    ms = timed(lambda: h.apply_device(x, y, batch=B, ld=n))
+    out["kernel"] = {"ms": ms, "GBps": algo / (ms / 1e3) / 1e9, "frac": algo / (ms / 1e3) / 1e9 / HBM_PEAK_GBS}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -642,7 +690,7 @@ def main() -> int:
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
-                             "trajectory", "archive_batch", "gradcheck", "pulse"),
+                             "trajectory", "archive_batch", "gradcheck", "pulse", "ceiling"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -690,6 +738,9 @@ def main() -> int:
         return 0
     if args.mode == "pulse":
         print(json.dumps(pulse_probe()), flush=True)
+        return 0
+    if args.mode == "ceiling":
+        print(json.dumps(ceiling_probe(h)), flush=True)
         return 0
     if args.operator == "random":
         h = workloads.full_random_system(seed=1).hamiltonian()      # the H·ψ measurement only

@@ -29,7 +29,8 @@
  * R/rpm_quantum_needle_jd_sweeps.py:466-546):
  *   rpm_operator_apply_pairs   .matvec with a per-vector electron 4x4 (field orientation/strength)
  *   rpm_lanczos_basis_batch    AdaptiveLanczosPropagator._basis        R/rpm_statevector_engine.py:651-688
- *   rpm_lanczos_basis_batch_prefix  the same, built in pieces            R/rpm_statevector_engine.py:651-688
+ *   rpm_lanczos_basis_batch_prefix  the same, built in pieces, optionally two steps per read of the basis
+ *                              R/rpm_statevector_engine.py:651-688
  *   rpm_krylov_tasks           _candidate / evaluate_projected         R/rpm_statevector_engine.py:690-715,531-538
  *   rpm_singlet_gram_batch     SingletPopulationObservable.project_basis R/rpm_statevector_engine.py:521-529
  *   rpm_krylov_combine_batch   endpoint update psi <- |psi| V c        R/rpm_statevector_engine.py:904
@@ -178,21 +179,31 @@ int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_sta
 
 /*
  * rpm_lanczos_basis_batch in pieces (same reference function, R/rpm_statevector_engine.py:651-688): the
- * first vectors of a basis now, the rest in a later call, bitwise equal to one uninterrupted build.
- * Trajectory b with d_targets[b] > 0 starts a new basis (eventual size min(d_targets[b], max_target))
- * from d_states + b*ld and builds its first d_stops[b] vectors (all of them when d_stops[b] <= 0); with
- * d_targets[b] == 0 and d_stops[b] > 0 it continues the basis an earlier call of this function left in
- * the same workspace and buffers, up to d_stops[b] vectors; otherwise it is left untouched.  Lanczos
- * steps j_begin <= j < max_stop are launched (j_begin no larger than the smallest size a continued basis
- * has, 0 when a basis starts; max_stop the largest d_stops).  info[4b] = vectors built so far (less
- * than d_stops[b] on breakdown), info[4b+1] = 1 if no breakdown, info[4b+2] = explicit corrections;
- * the tridiagonal of the built vectors is in d_alpha/d_beta as for the whole basis.
+ * first vectors of a basis now, the rest in a later call.  Trajectory b with d_targets[b] > 0 starts a
+ * new basis (eventual size min(d_targets[b], max_target)) from d_states + b*ld and builds its first
+ * d_stops[b] vectors (all of them when d_stops[b] <= 0); with d_targets[b] == 0 and d_stops[b] > 0 it
+ * continues the basis an earlier call of this function left in the same workspace and buffers, up to
+ * d_stops[b] vectors; otherwise it is left untouched.  Lanczos steps j_begin <= j < max_stop are
+ * launched (j_begin no larger than the smallest size a continued basis has, 0 when a basis starts;
+ * max_stop the largest d_stops).  info[4b] = vectors built so far (less than d_stops[b] on breakdown),
+ * info[4b+1] = 1 if no breakdown, info[4b+2] = explicit corrections, info[4b+3] = 1 if the call stopped
+ * the basis early (see flags); the tridiagonal of the built vectors is in d_alpha/d_beta as for the
+ * whole basis.  The workspace is rpm_lanczos_prefix_workspace_bytes(op, m, batch) bytes.
+ * flags = 0: each step reads the basis once (bitwise equal to one uninterrupted build).
+ * flags = RPM_LANCZOS_PAIRED: from step 12 on, steps j and j+1 (j even) share one read of the basis (the
+ * second step's projections formed by linearity from H applied to the unprojected three-term vector);
+ * every call still ends with its vectors materialised, so calls of either kind continue each other's
+ * bases.  A near-breakdown inside a pair stops that trajectory's basis early with info[4b+3] = 1 and
+ * info[4b+1] = 1: the vectors so far are valid, and a further call (d_targets[b] = 0) continues it.
+ * The result is deterministic per trajectory (independent of the other trajectories of the batch).
  */
+#define RPM_LANCZOS_PAIRED 1
+size_t rpm_lanczos_prefix_workspace_bytes(const rpm_operator* op, int m, int batch);
 int rpm_lanczos_basis_batch_prefix(const rpm_operator* op, int batch, const void* d_states, long long ld,
                                    const int* d_targets, int max_target, const int* d_stops, int j_begin,
                                    int max_stop, const void* d_pairs, void* d_basis, int m, double* d_alpha,
                                    double* d_beta, int* d_info, double* d_norm, double breakdown_tolerance,
-                                   void* d_workspace, size_t workspace_bytes, void* stream);
+                                   int flags, void* d_workspace, size_t workspace_bytes, void* stream);
 
 /*
  * Batched small-matrix evaluations.  d_tasks: ntasks records of 32 bytes

@@ -20,6 +20,7 @@ RPM_E_INVALID = -1
 RPM_E_HIP = -2
 RPM_E_NOMEM = -3
 RPM_E_UNSUPPORTED = -4
+RPM_LANCZOS_PAIRED = 1          # rpm_lanczos_basis_batch_prefix: two steps per read of the basis
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)
@@ -83,7 +84,8 @@ SIGNATURES = {
     "rpm_yield_accumulate": (ctypes.c_int, [_vp, _vp, _ll, _vp, ctypes.c_int, _vp, _ll, _vp, _vp, _vp]),
     "rpm_lanczos_basis_batch_prefix": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _ll, _vp, ctypes.c_int, _vp,
                                                       ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, _vp, _vp,
-                                                      _vp, _vp, ctypes.c_double, _vp, _sz, _vp]),
+                                                      _vp, _vp, ctypes.c_double, ctypes.c_int, _vp, _sz, _vp]),
+    "rpm_lanczos_prefix_workspace_bytes": (_sz, [_vp, ctypes.c_int, ctypes.c_int]),
     "rpm_adjoint_sweep": (ctypes.c_int, [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.c_double, ctypes.c_double,
                                          ctypes.c_int, _c_int_p, ctypes.c_int, _c_double_p, _vp, _c_int_p,
                                          _c_double_p, _vp, _c_int_p, ctypes.c_int, _vp, _c_int_p, ctypes.c_int,

@@ -133,6 +133,8 @@ class BatchedLanczosPropagator:
         # replay each basis step window as a HIP graph (captured once per propagate call)
         self.use_graphs = False
         self.graphs = None
+        # Lanczos steps from the 12th on in pairs that share one read of the basis
+        self.paired_steps = True
 
     @classmethod
     def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
@@ -363,7 +365,8 @@ class BatchedLanczosPropagator:
         beta = torch.zeros((B, m), dtype=torch.float64, device=dev)
         info = torch.zeros((B, 4), dtype=torch.int32, device=dev)
         norm = torch.zeros(B, dtype=torch.float64, device=dev)
-        ws = nat.workspace(lib.rpm_lanczos_batch_workspace_bytes(ctypes.c_void_p(op.handle), m, B))
+        ws = nat.workspace(lib.rpm_lanczos_prefix_workspace_bytes(ctypes.c_void_p(op.handle), m, B))
+        flags = nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0
         gram_ws = nat.workspace(lib.rpm_singlet_gram_batch_workspace_bytes(n // 4, B))
         P = torch.zeros((B, m, m), dtype=torch.complex128, device=dev)
         ladder_len = len(self._ladder(self.minimum_dimension))
@@ -447,13 +450,24 @@ class BatchedLanczosPropagator:
                             vp(op.handle), B, vp(nat.ptr(states)), n, vp(nat.ptr(d_ts)), m, vp(nat.ptr(d_ts) + 4 * B),
                             j0, max_stop, vp(nat.ptr(d_pairs) if d_pairs is not None else None), vp(nat.ptr(V)), m,
                             vp(nat.ptr(alpha)), vp(nat.ptr(beta)), vp(nat.ptr(info)), vp(nat.ptr(norm)),
-                            float(self.breakdown_tolerance), vp(nat.ptr(ws)), ws.numel(), vp(stream)))
+                            float(self.breakdown_tolerance), flags, vp(nat.ptr(ws)), ws.numel(), vp(stream)))
 
                     if graphs is not None:
                         graphs.run((j0, max_stop), launch)
                     else:
                         launch(nat.stream_ptr())
                     inf_h = info.cpu().numpy()
+                    while inf_h[group, 3].any():
+                        # paired steps stopped these bases early at a near-breakdown: continue them
+                        short = [b for b in group if inf_h[b, 3]]
+                        ts = np.zeros(2 * B, dtype=np.int32)
+                        for b in short:
+                            ts[B + b] = min(int(pending[b][1][1]), n)
+                        d_ts.copy_(torch.from_numpy(ts))
+                        launch(nat.stream_ptr(), j0=int(min(inf_h[b, 0] for b in short)), max_stop=int(ts[B:].max()))
+                        inf_new = info.cpu().numpy()
+                        for b in short:
+                            inf_h[b] = inf_new[b]
                     nrm_h = norm.cpu().numpy()
                     for b in group:
                         size_h[b] = int(inf_h[b, 0])

@@ -122,6 +122,7 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
         C.part = 1;
         C.begin = C.stop;
         C.stop = max(C.stop, min(st, C.target));
+        C.flags = 0;
       } else {
         C.part = 0;
         C.begin = C.stop;     // no step of this call is live for it
@@ -149,6 +150,9 @@ __global__ void __launch_bounds__(kThreads) k_basis_start(const double* __restri
     C.begin = 0;
     C.stop = live ? first : 0;
     C.part = 1;
+    C.npend = 1;
+    C.zscale = 0.0;
+    C.flags = 0;
   }
 }
 
@@ -384,40 +388,30 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_pass(c128* __restrict__ V, lo
 // or |g| not small against |y|), every block subtracts its share of V_<=j g from y, and the last
 // block of the trajectory to finish (device-scope counter; the fences run on this rare path only)
 // takes beta_j from the corrected vector and clears c_{j+1}.
-__global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict__ V, long long vstride, long long n,
-                                                          const double* __restrict__ ysum,
-                                                          const c128* __restrict__ h, int hstride,
-                                                          c128* __restrict__ cpend, int cstride,
-                                                          const double* __restrict__ atilde, double* __restrict__ alpha,
-                                                          double* __restrict__ beta, int astride, int j, double tol,
-                                                          c128* __restrict__ U, double* __restrict__ fpart,
-                                                          unsigned* __restrict__ counter, BasisCtl* ctl) {
-  const int b = blockIdx.y;
-  if (!basis_live(ctl, b, j, 0)) return;
-  __shared__ double sm[kThreads / kWave];
-  __shared__ c128 gs[kMaxBasis];
-  __shared__ int is_last;
+// Every block reads the control block before the lead block updates it, or decides only what the
+// lead block also decides from h (whether the correction is due): nothing a block branches on
+// depends on the order in which the blocks run.
+__device__ inline void finish_step(const c128* __restrict__ Vb, long long n, double yy, const c128* __restrict__ hb,
+                                   c128* __restrict__ cb, double at,
+                                   double* __restrict__ alpha_b, double* __restrict__ beta_b, int j, double tol,
+                                   c128* __restrict__ y, double* __restrict__ fpart_b, unsigned* __restrict__ counter_b,
+                                   BasisCtl& C, double* sm, c128* gs, int* is_last) {
   const int t = threadIdx.x;
-  const c128* hb = h + b * hstride;
-  c128* cb = cpend + b * cstride;
-  BasisCtl& C = ctl[b];
-  const double yy = ysum[b];
+  const double bprev = C.beta_prev;
   double gg = 0.0;
   if (t < j) {
     const c128 v = hb[t];
     gg = v.re * v.re + v.im * v.im;
     gs[t] = v;
   }
-  const double at = atilde[b];
   const c128 gam = hb[j];
   const double cprev = j > 0 ? cb[j - 1].re : 0.0;
-  const double bprev = C.beta_prev;
   const int target = C.target;
   gg = block_sum(gg, sm);   // ends with a barrier: every read above precedes the writes below
   const c128 glast = cmake(gam.re - at, gam.im);
   gg += glast.re * glast.re + glast.im * glast.im;
   const bool lead = blockIdx.x == 0;
-  if (lead && t == 0) alpha[b * astride + j] = gam.re - bprev * cprev;
+  if (lead && t == 0) alpha_b[j] = gam.re - bprev * cprev;
   if (j + 1 >= target) return;                 // the trajectory's last vector: no beta_j
   const double b2 = yy - gg;
   bool fix = !(b2 > kCancel * yy);
@@ -425,7 +419,7 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   if (!fix) {
     if (!lead) return;
     const double bj = std::sqrt(b2);
-    if (t == 0) beta[b * astride + j] = bj;
+    if (t == 0) beta_b[j] = bj;
     if (bj <= tol) {
       if (t == 0) {
         C.active = 0;
@@ -446,8 +440,6 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   if (t == 0) gs[j] = glast;
   __syncthreads();
   const int k = j + 1;
-  const c128* Vb = V + b * vstride;
-  c128* y = U + b * n;
   double s = 0.0;
   for (long long i = static_cast<long long>(blockIdx.x) * kThreads + t; i < n;
        i += static_cast<long long>(gridDim.x) * kThreads) {
@@ -458,23 +450,23 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   }
   s = block_sum(s, sm);
   if (t == 0) {
-    fpart[static_cast<long long>(b) * gridDim.x + blockIdx.x] = s;
+    fpart_b[blockIdx.x] = s;
     __threadfence();
-    is_last = atomicAdd(&counter[b], 1u) == gridDim.x - 1;
+    *is_last = atomicAdd(counter_b, 1u) == gridDim.x - 1;
   }
   __syncthreads();
-  if (!is_last) return;
+  if (!*is_last) return;
   __threadfence();
-  const volatile double* fp = fpart + static_cast<long long>(b) * gridDim.x;
+  const volatile double* fp = fpart_b;
   double tot = 0.0;
   for (int i = t; i < static_cast<int>(gridDim.x); i += kThreads) tot += fp[i];
   tot = block_sum(tot, sm);
   if (t < k) cb[t] = cmake(0.0, 0.0);
   if (t == 0) {
-    counter[b] = 0;
+    *counter_b = 0;
     C.nfix += 1;
     const double bj = std::sqrt(tot);
-    beta[b * astride + j] = bj;
+    beta_b[j] = bj;
     if (bj <= tol) {
       C.active = 0;
       C.size = j + 1;
@@ -485,14 +477,728 @@ __global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict
   }
 }
 
+__global__ void __launch_bounds__(kThreads) k_lcgs_finish(const c128* __restrict__ V, long long vstride, long long n,
+                                                          const double* __restrict__ ysum,
+                                                          const c128* __restrict__ h, int hstride,
+                                                          c128* __restrict__ cpend, int cstride,
+                                                          const double* __restrict__ atilde, double* __restrict__ alpha,
+                                                          double* __restrict__ beta, int astride, int j, double tol,
+                                                          c128* __restrict__ U, double* __restrict__ fpart,
+                                                          unsigned* __restrict__ counter, BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  if (!basis_live(ctl, b, j, 0)) return;
+  __shared__ double sm[kThreads / kWave];
+  __shared__ c128 gs[kMaxBasis];
+  __shared__ int is_last;
+  finish_step(V + b * vstride, n, ysum[b], h + b * hstride, cpend + b * cstride, atilde[b],
+              alpha + b * astride, beta + b * astride, j, tol, U + b * n, fpart + static_cast<long long>(b) * gridDim.x,
+              counter + b, ctl[b], sm, gs, &is_last);
+}
+
+// ---------------------------------------------------------------- paired steps
+//
+// Steps j and j+1 (j even) share one read of V. With x_j pending (v_j = s x_j - V_<j c_B) and,
+// after an earlier pair, v_{j-1} pending as well (v_{j-1} = a z' - V_<j-1 c_A for the previous
+// pair's z'), write x^_j = s x_j and x^_{j-1} = a z' (or the materialised v_{j-1}):
+//   M1     P = H x^_j and the partials of alpha~_j = <x^_j, P>;
+//   prep   z = P - beta_{j-1} x^_{j-1} - alpha~_j x^_j: the three-term step on unprojected vectors,
+//          so that z = y_j + (a vector of span V_<j with known coefficients);
+//   M2     Q = H z and the partials of <z, Q>;
+//   pass   one read of the materialised columns: v_{j-1} and v_j materialised, V^H r (r = z +
+//          alpha~_j x^_j = u_j less beta_{j-1} V c_A) and V^H Q, and in-block sums of the columns
+//          materialised here and of <v_j, u_j>, |y_j|^2, <v_j, Q>, <z, v_j>, |v_j|^2, |Q|^2, |z|^2;
+//   finish step j as a single step (alpha_j, beta_j, g_j = [D; gamma_j - alpha~_j]); then
+//          v_{j+1} = z / beta_j - V_<=j c_A' with c_A' = (g_j - alpha~_j c_B) / beta_j (D here
+//          without the beta_{j-1} c_A part), and step j+1 by linearity: H x^_{j+1} = Q / beta_j for
+//          x^_{j+1} = z / beta_j, V^H u_{j+1} = V^H Q / beta_j - beta_j e_j, gamma_{j+1} =
+//          <x^_{j+1}, u_{j+1}> - c_A'^H V^H u_{j+1}, alpha~_{j+1} = <z, Q> / beta_j^2 and |y'|^2 of
+//          y' = u_{j+1} - alpha~_{j+1} x^_{j+1} from the in-block sums; then alpha_{j+1}, beta_{j+1}
+//          and c_B' as in a single step;
+//   close  x_{j+2} = y' = Q / beta_j - beta_j v_j - alpha~_{j+1} / beta_j z, pending with c_B';
+//          v_{j+1} stays pending (z, c_A') until the next pass materialises it.
+// The explicit correction (|c_A'| or |c_B'| above kPairFixThreshold, cancellation in beta_{j+1}^2, and
+// the last pair of a call, which leaves v_{j+1} materialised and one vector pending) materialises
+// v_{j+1}, subtracts V_<=j+1 g_{j+1} from x_{j+2} and takes beta_{j+1} from its norm. Cancellation
+// in beta_j^2 (a near-breakdown at step j, which a single step corrects before the next matvec)
+// stops the call's basis at j + 1 with the corrected vector pending (kBasisShort): the caller
+// continues it with another call.
+// In the same launches, a trajectory whose window holds only step j (S0) takes a single step on P;
+// one whose window holds only step j+1 (S1) takes a single step on M2 applied to its own x_{j+1}.
+// Pending vectors: x_p in w[p % 2] (as in single steps); P in w[1]; z alternates between two
+// buffers; Q has its own.
+// Lanczos index of the first pair. Early steps read few basis columns, so the pair's own traffic
+// (prep, close and the extra staged vectors: about eight vectors per trajectory) outweighs the column
+// read it saves. Measured on 32 production trajectories over 0.5 us: first pair at 2, 4, 8, 12, 16,
+// 20 gives 37.6, 37.9, 38.0, 38.1, 37.6, 36.4 trajectory-us/s.
+constexpr int kPairFrom = 12;
+// |c_A'|, |c_B'| above which a pair closes with the explicit correction. A pair's pending
+// coefficients carry the earlier ones times alpha~ and beta (z = y_j - beta_{j-1} V c_A - alpha~_j
+// V c_B), about |H| / beta larger than a single step's for the same lag, so the bound is that
+// much looser than kFixThreshold for the same size of lagged component.
+constexpr double kPairFixThreshold = 1e-4;
+constexpr int kColE = kMaxBasis;             // partial / sum columns of V^H Q
+constexpr int kColS = 2 * kMaxBasis;         // in-block scalar sums: (|y|^2, |v|^2), (|Q|^2, |z|^2), <z, v>
+constexpr int kPairCols = 2 * kMaxBasis + 8; // columns of the pass partials and sums
+constexpr int kCloseGamma = 1, kCloseFix = 2, kCloseTarget = 4, kCloseRetreat = 8, kCloseMask = 15;
+static_assert(kBasisShort > kCloseMask, "flag bits overlap");
+enum { kModeIdle = 0, kModePair = 1, kModeS0 = 2, kModeS1 = 3 };
+
+__device__ inline int pair_mode(const BasisCtl& C, int j) {
+  const bool l0 = basis_step_live(C, j), l1 = basis_step_live(C, j + 1);
+  return l0 ? (l1 ? kModePair : kModeS0) : (l1 ? kModeS1 : kModeIdle);
+}
+
+// Input of M2: z / s for a pair (M2 scales its input by s, so it forms H z), x_{j+1} for S1.
+__global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__ V, long long vstride, long long n,
+                                                        int j, const c128* __restrict__ X0, const c128* __restrict__ X1,
+                                                        const c128* __restrict__ ZA, c128* __restrict__ Z,
+                                                        const double* __restrict__ atilde, const BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  const BasisCtl& C = ctl[b];
+  const int mode = pair_mode(C, j);
+  if (mode != kModePair && mode != kModeS1) return;
+  const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
+  if (i >= n) return;
+  const long long o = b * n + i;
+  if (mode == kModeS1) {
+    Z[o] = X1[o];
+    return;
+  }
+  const double s = C.scale, at = atilde[b], bp = C.beta_prev;
+  const c128 xh = cscale(X0[o], s);
+  const c128 xp = C.npend == 2 ? cscale(ZA[o], C.zscale) : V[b * vstride + static_cast<long long>(j - 1) * n + i];
+  c128 z = X1[o];
+  z.re = fma(-bp, xp.re, z.re);
+  z.im = fma(-bp, xp.im, z.im);
+  z.re = fma(-at, xh.re, z.re);
+  z.im = fma(-at, xh.im, z.im);
+  Z[o] = cscale(z, 1.0 / s);
+}
+
+struct PairPass {
+  c128* Vb;                 // this trajectory's basis
+  long long n, x0;
+  int p, cnt, blk, nblk;
+  const c128* xb;           // pending x_p (scale s)
+  const c128* za;           // pending z of v_{p-1} (scale a), two pending vectors only
+  const c128* r;            // pair: z / s; single: H x^_p
+  const c128* q;            // pair: Q
+  const c128* cA;           // pending coefficients of v_{p-1} (p - 1 entries) and of v_p (p entries)
+  const c128* cB;
+  c128* y;                  // single: y = u - alpha~ v_p (the next pending vector)
+  c128* part;               // [kPairCols][nblk] partial sums
+  double s, a, at, bp;
+};
+
+// One block of the paired pass (thread t owns element x0 + t outside the column stream).
+template <bool kStream, bool kFull, bool kPair, bool kNp2>
+__device__ inline void pair_pass_body(const PairPass& P, c128* s_r, c128* s_q, c128* s_x, c128* s_a, c128* s_z,
+                                      c128 (*s_corr)[kPassElems], c128* s_cA, c128* s_cB, double* s_red) {
+  const int t = threadIdx.x, lane = t % kWave, wq = t / kWave;
+  const long long n = P.n, x0 = P.x0;
+  const int p = P.p, cnt = P.cnt;
+  const int mc = kNp2 ? p - 1 : p;           // materialised columns, streamed
+  const bool mine = t < cnt;
+  const int et = mine ? t : cnt - 1;
+  const int ncol = mc > wq ? (mc - wq + kPassWaves - 1) / kPassWaves : 0;
+  c128 va[kPassPer], vb[kPassPer];
+  if (ncol > 0) pass_load<kStream, kFull>(P.Vb + static_cast<long long>(wq) * n + x0, lane, cnt, va);
+  {
+    // staged: x^_p, a z' (two pending vectors), z (pairs), r and Q; elements past cnt weigh nothing
+    const c128 xh = cscale(P.xb[x0 + et], P.s);
+    s_x[t] = xh;
+    c128 xa = cmake(0.0, 0.0);
+    if constexpr (kNp2) {
+      xa = cscale(P.za[x0 + et], P.a);
+      s_a[t] = xa;
+    }
+    c128 r1;
+    if constexpr (kPair) {
+      const c128 z = cscale(P.r[x0 + et], P.s);
+      s_z[t] = z;
+      r1 = z;
+      r1.re = fma(P.at, xh.re, r1.re);
+      r1.im = fma(P.at, xh.im, r1.im);
+      s_q[t] = mine ? P.q[x0 + et] : cmake(0.0, 0.0);
+    } else {
+      const c128 xp = kNp2 ? xa : P.Vb[static_cast<long long>(p - 1) * n + x0 + et];
+      r1 = P.r[x0 + et];
+      r1.re = fma(-P.bp, xp.re, r1.re);
+      r1.im = fma(-P.bp, xp.im, r1.im);
+    }
+    s_r[t] = mine ? r1 : cmake(0.0, 0.0);
+  }
+  if (t < p) s_cB[t] = P.cB[t];
+  if (kNp2 && t < p - 1) s_cA[t] = P.cA[t];
+  __syncthreads();
+  c128 corrB[kPassPer], corrA[kPassPer];
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) {
+    corrB[it] = cmake(0.0, 0.0);
+    corrA[it] = cmake(0.0, 0.0);
+  }
+  auto load = [&](int k, c128(&v)[kPassPer]) {
+    pass_load<kStream, kFull>(P.Vb + static_cast<long long>(wq + kPassWaves * k) * n + x0, lane, cnt, v);
+  };
+  auto use = [&](int k, const c128(&v)[kPassPer]) {
+    const int c = wq + kPassWaves * k;
+    const c128 cb = s_cB[c];
+    const c128 cav = kNp2 ? s_cA[c] : cmake(0.0, 0.0);
+    c128 accD = cmake(0.0, 0.0), accE = cmake(0.0, 0.0);
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) {
+      accD = cfmaconj(v[it], s_r[it * kWave + lane], accD);
+      if constexpr (kPair) accE = cfmaconj(v[it], s_q[it * kWave + lane], accE);
+      corrB[it] = cfma(v[it], cb, corrB[it]);
+      if constexpr (kNp2) corrA[it] = cfma(v[it], cav, corrA[it]);
+    }
+    accD = wave_sum(accD);
+    if constexpr (kPair) accE = wave_sum(accE);
+    if (lane == 0) {
+      P.part[static_cast<long long>(c) * P.nblk + P.blk] = accD;
+      if constexpr (kPair) P.part[static_cast<long long>(kColE + c) * P.nblk + P.blk] = accE;
+    }
+  };
+  int k = 0;
+  for (; k + 2 < ncol; k += 2) {
+    load(k + 1, vb);
+    use(k, va);
+    load(k + 2, va);
+    use(k + 1, vb);
+  }
+  if (k + 1 < ncol) {
+    load(k + 1, vb);
+    use(k, va);
+    use(k + 1, vb);
+  } else if (k < ncol) {
+    use(k, va);
+  }
+  // corrections summed over the waves in a fixed order
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) s_corr[wq][it * kWave + lane] = corrB[it];
+  __syncthreads();
+  c128 sB = cadd(cadd(cadd(s_corr[0][t], s_corr[1][t]), s_corr[2][t]), s_corr[3][t]);
+  c128 sA = cmake(0.0, 0.0);
+  if constexpr (kNp2) {
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) s_corr[wq][it * kWave + lane] = corrA[it];
+    __syncthreads();
+    sA = cadd(cadd(cadd(s_corr[0][t], s_corr[1][t]), s_corr[2][t]), s_corr[3][t]);
+  }
+  // per element: v_{p-1}, v_p, u_p, y_p and the in-block sums
+  constexpr int K = 14;
+  double red[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) red[i] = 0.0;
+  if (mine) {
+    const long long e = x0 + t;
+    const c128 r1 = s_r[t];
+    c128 u = r1;
+    c128 vA = cmake(0.0, 0.0);
+    if constexpr (kNp2) {
+      vA = csub(s_a[t], sA);
+      P.Vb[static_cast<long long>(p - 1) * n + e] = vA;
+      sB = cfma(vA, s_cB[p - 1], sB);
+      u.re = fma(P.bp, sA.re, u.re);
+      u.im = fma(P.bp, sA.im, u.im);
+    }
+    const c128 vB = csub(s_x[t], sB);
+    P.Vb[static_cast<long long>(p) * n + e] = vB;
+    const c128 y = cmake(fma(-P.at, vB.re, u.re), fma(-P.at, vB.im, u.im));
+    const c128 g = cfmaconj(vB, u, cmake(0.0, 0.0));
+    red[0] = g.re;
+    red[1] = g.im;
+    red[2] = fma(y.re, y.re, y.im * y.im);
+    if constexpr (kNp2) {
+      const c128 d = cfmaconj(vA, u, cmake(0.0, 0.0));
+      red[3] = d.re;
+      red[4] = d.im;
+    }
+    if constexpr (kPair) {
+      const c128 q = s_q[t];
+      const c128 z = s_z[t];
+      const c128 eb = cfmaconj(vB, q, cmake(0.0, 0.0));
+      const c128 zv = cfmaconj(z, vB, cmake(0.0, 0.0));
+      red[5] = eb.re;
+      red[6] = eb.im;
+      red[7] = zv.re;
+      red[8] = zv.im;
+      red[9] = fma(vB.re, vB.re, vB.im * vB.im);
+      red[10] = fma(q.re, q.re, q.im * q.im);
+      red[11] = fma(z.re, z.re, z.im * z.im);
+      if constexpr (kNp2) {
+        const c128 dq = cfmaconj(vA, q, cmake(0.0, 0.0));
+        red[12] = dq.re;
+        red[13] = dq.im;
+      }
+    } else {
+      P.y[e] = y;
+    }
+  }
+  // block sums: each wave, then the waves in order
+#pragma unroll
+  for (int i = 0; i < K; ++i) red[i] = wave_sum(red[i]);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) s_red[wq * K + i] = red[i];
+  }
+  __syncthreads();
+  if (t == 0) {
+    double tot[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) tot[i] = s_red[i];
+    for (int w = 1; w < kPassWaves; ++w) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) tot[i] += s_red[w * K + i];
+    }
+    auto put = [&](int col, double re, double im) { P.part[static_cast<long long>(col) * P.nblk + P.blk] = cmake(re, im); };
+    put(p, tot[0], tot[1]);
+    if constexpr (kNp2) put(p - 1, tot[3], tot[4]);
+    if constexpr (kPair) {
+      put(kColE + p, tot[5], tot[6]);
+      if constexpr (kNp2) put(kColE + p - 1, tot[12], tot[13]);
+      put(kColS, tot[2], tot[9]);
+      put(kColS + 1, tot[10], tot[11]);
+      put(kColS + 2, tot[7], tot[8]);
+    } else {
+      put(kColS, tot[2], 0.0);
+    }
+  }
+}
+
+template <bool kStream, bool kFull>
+__global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, long long vstride, long long n, int j,
+                                                        c128* __restrict__ X0, c128* __restrict__ X1,
+                                                        const c128* __restrict__ ZA, const c128* __restrict__ Z,
+                                                        const c128* __restrict__ Q, const c128* __restrict__ cpendA,
+                                                        const c128* __restrict__ cpend, int cstride,
+                                                        const double* __restrict__ atilde,
+                                                        const double* __restrict__ atilde2, c128* __restrict__ partial,
+                                                        long long pstride, const BasisCtl* ctl, int blk0, int nblk) {
+  static_assert(kPassElems == kThreads, "one element per thread outside the column stream");
+  const int b = blockIdx.y;
+  const BasisCtl& C = ctl[b];
+  const int mode = pair_mode(C, j);
+  if (mode == kModeIdle) return;
+  __shared__ c128 s_r[kPassElems];
+  __shared__ c128 s_q[kPassElems];
+  __shared__ c128 s_x[kPassElems];
+  __shared__ c128 s_a[kPassElems];
+  __shared__ c128 s_z[kPassElems];
+  __shared__ c128 s_corr[kPassWaves][kPassElems];
+  __shared__ c128 s_cA[kMaxBasis];
+  __shared__ c128 s_cB[kMaxBasis];
+  __shared__ double s_red[kPassWaves * 14];
+  PairPass P;
+  P.Vb = V + b * vstride;
+  P.n = n;
+  P.blk = blk0 + blockIdx.x;
+  P.nblk = nblk;
+  P.x0 = static_cast<long long>(P.blk) * kPassElems;
+  P.cnt = kFull ? kPassElems : static_cast<int>(n - P.x0 < kPassElems ? n - P.x0 : kPassElems);
+  P.s = C.scale;
+  P.a = C.zscale;
+  P.bp = C.beta_prev;
+  P.cA = cpendA + b * cstride;
+  P.cB = cpend + b * cstride;
+  P.za = ZA + b * n;
+  P.part = partial + b * pstride;
+  P.q = Q + b * n;
+  P.y = nullptr;
+  const bool np2 = C.npend == 2;
+  if (mode == kModePair) {
+    P.p = j;
+    P.xb = X0 + b * n;
+    P.r = Z + b * n;
+    P.at = atilde[b];
+    if (np2)
+      pair_pass_body<kStream, kFull, true, true>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
+    else
+      pair_pass_body<kStream, kFull, true, false>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
+  } else if (mode == kModeS0) {
+    P.p = j;
+    P.xb = X0 + b * n;
+    P.r = X1 + b * n;      // P = H x^_j
+    P.y = X1 + b * n;      // y_j, the pending x_{j+1}, over P (each element read before it is written)
+    P.at = atilde[b];
+    if (np2)
+      pair_pass_body<kStream, kFull, false, true>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
+    else
+      pair_pass_body<kStream, kFull, false, false>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
+  } else {
+    P.p = j + 1;           // a window that starts at j + 1 follows a call that left one vector pending
+    P.xb = X1 + b * n;
+    P.r = Q + b * n;       // M2 on x_{j+1}
+    P.y = X0 + b * n;
+    P.at = atilde2[b];
+    pair_pass_body<kStream, kFull, false, false>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
+  }
+}
+
+// h[b][col] = sum_blk partial[b][col][blk] over the slot's columns: 0..p, the in-block scalars, and
+// (pairs) kColE..kColE + p. Grid (2j + 6, batch). A single step whose pass materialised v_{p-1}
+// from a pending z gets D[c] = V_c^H u = (the streamed V_c^H r) + beta_{p-1} cA[c] (c < p - 1); a
+// pair's k_pair_finish forms that sum itself.
+__global__ void __launch_bounds__(kThreads) k_pair_reduce(const c128* __restrict__ partial, int nblk, long long pstride,
+                                                          c128* __restrict__ h, int hstride, int j,
+                                                          const c128* __restrict__ cpendA, int cstride,
+                                                          const BasisCtl* ctl) {
+  const int b = blockIdx.y;
+  const BasisCtl& C = ctl[b];
+  const int mode = pair_mode(C, j);
+  if (mode == kModeIdle) return;
+  const int p = mode == kModeS1 ? j + 1 : j;
+  const int x = blockIdx.x;
+  int col;
+  if (x <= p)
+    col = x;
+  else if (x <= p + 3)
+    col = kColS + (x - p - 1);
+  else if (mode == kModePair && x - (p + 4) <= p)
+    col = kColE + (x - p - 4);
+  else
+    return;
+  __shared__ c128 sm[kThreads / kWave];
+  const c128* pp = partial + b * pstride + static_cast<long long>(col) * nblk;
+  c128 s{0.0, 0.0};
+  for (int i = threadIdx.x; i < nblk; i += kThreads) s = cadd(s, pp[i]);
+  s = block_sum(s, sm);
+  if (threadIdx.x == 0) {
+    if (mode == kModeS0 && C.npend == 2 && col < p - 1) {
+      const c128 a = cpendA[b * cstride + col];
+      s.re = fma(C.beta_prev, a.re, s.re);
+      s.im = fma(C.beta_prev, a.im, s.im);
+    }
+    h[b * hstride + col] = s;
+  }
+}
+
+// Scalars of the slot (see above). Single steps (S0, S1) as k_lcgs_finish, with the correction of a
+// pending v_{p-1} added to D; a pair's scalars on block 0, its vector work left to k_pair_close.
+// psc[b][0..4]: coefficients of x_{j+2} on Q, v_j and the stored z / s, the scale of the stored z
+// for v_{j+1}, and s.
+__global__ void __launch_bounds__(kThreads) k_pair_finish(const c128* __restrict__ V, long long vstride, long long n,
+                                                          const c128* __restrict__ h, int hstride,
+                                                          c128* __restrict__ cpendA, c128* __restrict__ cpend,
+                                                          c128* __restrict__ gfix, int cstride,
+                                                          const double* __restrict__ atilde,
+                                                          const double* __restrict__ atilde2, double* __restrict__ alpha,
+                                                          double* __restrict__ beta, int astride, int j, double tol,
+                                                          c128* __restrict__ X0, c128* __restrict__ X1,
+                                                          double* __restrict__ fpart, unsigned* __restrict__ counter,
+                                                          BasisCtl* ctl, double* __restrict__ psc) {
+  const int b = blockIdx.y;
+  BasisCtl& C = ctl[b];
+  const int mode = pair_mode(C, j);
+  if (mode == kModeIdle) return;
+  __shared__ double sm[kThreads / kWave];
+  __shared__ c128 smc[kThreads / kWave];
+  __shared__ c128 gs[kMaxBasis + 2];
+  __shared__ c128 ca1[kMaxBasis + 2];
+  __shared__ int is_last;
+  const c128* hb = h + b * hstride;
+  if (mode != kModePair) {
+    const int p = mode == kModeS1 ? j + 1 : j;
+    finish_step(V + b * vstride, n, hb[kColS].re, hb, cpend + b * cstride,
+                mode == kModeS1 ? atilde2[b] : atilde[b], alpha + b * astride, beta + b * astride, p, tol,
+                (mode == kModeS1 ? X0 : X1) + b * n, fpart + static_cast<long long>(b) * gridDim.x, counter + b, C, sm,
+                gs, &is_last);
+    return;
+  }
+  if (blockIdx.x != 0) return;
+  const int t = threadIdx.x;
+  const bool np2 = C.npend == 2;
+  const double bp = C.beta_prev, s = C.scale, at = atilde[b], zq = atilde2[b];
+  c128* ca = cpendA + b * cstride;
+  c128* cb = cpend + b * cstride;
+  c128* gf = gfix + b * cstride;
+  double* ps = psc + b * 8;
+  const double yy = hb[kColS].re, vv = hb[kColS].im, qq = hb[kColS + 1].re, zz = hb[kColS + 1].im;
+  const c128 zv = hb[kColS + 2];
+  // step j: D (with the pending v_{j-1} correction), alpha_j, beta_j
+  double gg = 0.0;
+  c128 graw = cmake(0.0, 0.0);     // g_j without the beta_{j-1} c_A part, less alpha~_j c_B
+  if (t < j) {
+    const c128 hv = hb[t];
+    c128 v = hv;
+    if (np2 && t < j - 1) {
+      const c128 a = ca[t];
+      v.re = fma(bp, a.re, v.re);
+      v.im = fma(bp, a.im, v.im);
+    }
+    gg = v.re * v.re + v.im * v.im;
+    const c128 c = cb[t];
+    graw = cmake(fma(-at, c.re, hv.re), fma(-at, c.im, hv.im));
+  }
+  const c128 gam = hb[j];
+  const double cprev = cb[j - 1].re;
+  const c128 ej = hb[kColE + j];
+  c128 ev = t <= j ? hb[kColE + t] : cmake(0.0, 0.0);
+  gg = block_sum(gg, sm);          // ends with a barrier: the reads above precede the writes below
+  const c128 glast = cmake(gam.re - at, gam.im);
+  gg += glast.re * glast.re + glast.im * glast.im;
+  const double aj = gam.re - bp * cprev;
+  if (t == 0) alpha[b * astride + j] = aj;
+  const double b2 = yy - gg;
+  if (!(b2 > kCancel * yy)) {
+    // near-breakdown at step j: k_pair_close corrects z and stops the basis at j + 1
+    if (t < j) gf[t] = graw;
+    if (t == 0) {
+      gf[j] = glast;
+      ps[4] = s;
+      C.flags = (C.flags & ~kCloseMask) | kCloseRetreat;
+    }
+    return;
+  }
+  const double bj = std::sqrt(b2);
+  if (t == 0) beta[b * astride + j] = bj;
+  if (bj <= tol) {
+    if (t == 0) {
+      C.active = 0;
+      C.size = j + 1;
+    }
+    return;
+  }
+  const double ib = 1.0 / bj;
+  if (t < j) ca1[t] = cscale(graw, ib);
+  if (t == 0) ca1[j] = cscale(glast, ib);
+  __syncthreads();
+  // step j+1 by linearity
+  c128 d1 = cmake(0.0, 0.0);
+  if (t <= j) {
+    d1 = cscale(ev, ib);
+    if (t == j) d1.re = fma(-bj, vv, d1.re);
+  }
+  const double at1 = zq * ib * ib;
+  const c128 cv = t <= j ? ca1[t] : cmake(0.0, 0.0);
+  const c128 dot = block_sum(cfmaconj(cv, d1, cmake(0.0, 0.0)), smc);
+  const c128 gam1 = cmake(at1 - zv.re - dot.re, -zv.im - dot.im);
+  if (t == 0) alpha[b * astride + j + 1] = gam1.re - bj * ca1[j].re;
+  const double zsc = s * ib;
+  if (t == 0) {
+    ps[0] = ib;
+    ps[1] = bj;
+    ps[2] = at1 * s * ib;
+    ps[3] = zsc;
+  }
+  if (t <= j) ca[t] = cv;
+  if (j + 2 >= C.target) {
+    // v_{j+1} is the basis's last vector: k_pair_close materialises it
+    if (t == 0) C.flags = (C.flags & ~kCloseMask) | kCloseTarget;
+    return;
+  }
+  c128 g1 = cmake(0.0, 0.0);
+  if (t <= j) g1 = cmake(fma(-at1, cv.re, d1.re), fma(-at1, cv.im, d1.im));
+  if (t == j + 1) g1 = cmake(gam1.re - at1, gam1.im);
+  const double gg1 = block_sum(g1.re * g1.re + g1.im * g1.im, sm);
+  const double caa = block_sum(cv.re * cv.re + cv.im * cv.im, sm);
+  const double yy1 = qq * ib * ib + bj * bj * vv + at1 * at1 * zz * ib * ib - 2.0 * ej.re - 2.0 * at1 * zq * ib * ib +
+                     2.0 * at1 * zv.re;
+  const double b21 = yy1 - gg1;
+  const double thr2 = kPairFixThreshold * kPairFixThreshold;
+  bool fix = j + 2 >= C.stop || !(b21 > kCancel * yy1) || gg1 > thr2 * b21 || caa > thr2;
+  const double b1 = fix ? 0.0 : std::sqrt(b21);
+  if (!fix && b1 <= tol) fix = true;
+  if (fix) {
+    if (t <= j + 1) gf[t] = g1;
+    if (t == 0) C.flags = (C.flags & ~kCloseMask) | kCloseFix;
+    return;
+  }
+  const double i1 = 1.0 / b1;
+  if (t <= j + 1) cb[t] = cscale(g1, i1);
+  if (t == 0) {
+    beta[b * astride + j + 1] = b1;
+    C.scale = i1;
+    C.beta_prev = b1;
+    C.zscale = zsc;
+    C.npend = 2;
+    C.flags = (C.flags & ~kCloseMask) | kCloseGamma;
+  }
+}
+
+// Vector work of a pair after k_pair_finish; single steps of the slot leave one vector pending.
+// Grid (blocks of kPassElems elements, batch), in one launch, or in two (kFull, then the last
+// partial block) that run in stream order: the last block of a trajectory to finish (device-scope
+// counter over all npass blocks) takes the explicit norm. Three-term close: x_{j+2} elementwise.
+// Explicit closes stream V_<=j as the pass does (waves over columns, the next column's loads in
+// flight) for the corrections V_<=j c_A' (v_{j+1}) and V_<=j g (x_{j+2} or, for a retreat, z).
+template <bool kStream, bool kFull>
+__global__ void __launch_bounds__(kThreads) k_pair_close(c128* __restrict__ V, long long vstride, long long n, int j,
+                                                         const c128* __restrict__ Z, const c128* __restrict__ Q,
+                                                         c128* __restrict__ X0, c128* __restrict__ X1,
+                                                         const c128* __restrict__ cpendA, c128* __restrict__ cpend,
+                                                         const c128* __restrict__ gfix, int cstride,
+                                                         const double* __restrict__ psc, double* __restrict__ beta,
+                                                         int astride, double tol, double* __restrict__ fpart,
+                                                         unsigned* __restrict__ counter, BasisCtl* ctl, int blk0,
+                                                         int nblk) {
+  const int b = blockIdx.y;
+  BasisCtl& C = ctl[b];
+  const int mode = pair_mode(C, j);
+  const int blk = blk0 + blockIdx.x;
+  if (mode == kModeS0 || mode == kModeS1) {
+    if (blk == 0 && threadIdx.x == 0) {
+      C.npend = 1;
+      C.flags &= ~kCloseMask;
+    }
+    return;
+  }
+  if (mode != kModePair) return;
+  const int cm = C.flags & kCloseMask;
+  if (cm == 0) return;
+  const int t = threadIdx.x, lane = t % kWave, wq = t / kWave;
+  const long long x0 = static_cast<long long>(blk) * kPassElems;
+  const int cnt = kFull ? kPassElems : static_cast<int>(n - x0 < kPassElems ? n - x0 : kPassElems);
+  const bool mine = t < cnt;
+  const long long e = x0 + (mine ? t : cnt - 1);
+  const double* ps = psc + b * 8;
+  const double kq = ps[0], kv = ps[1], kz = ps[2], zsc = ps[3], sz = ps[4];
+  c128* Vb = V + b * vstride;
+  const c128 zv = Z[b * n + e];
+  if (cm == kCloseGamma) {
+    if (mine) {
+      const c128 vj = Vb[static_cast<long long>(j) * n + e];
+      c128 x = cscale(Q[b * n + e], kq);
+      x = cmake(fma(-kv, vj.re, x.re), fma(-kv, vj.im, x.im));
+      x = cmake(fma(-kz, zv.re, x.re), fma(-kz, zv.im, x.im));
+      X0[b * n + e] = x;
+    }
+    return;
+  }
+  __shared__ c128 sa[kMaxBasis + 2];
+  __shared__ c128 sg[kMaxBasis + 2];
+  __shared__ c128 s_corr[kPassWaves][kPassElems];
+  __shared__ double sm[kThreads / kWave];
+  __shared__ int is_last;
+  const bool needA = cm != kCloseRetreat, needG = cm != kCloseTarget;
+  // (a retreat has no c_A', a target close no g: their products are computed and not used)
+  if (t <= j) sa[t] = needA ? cpendA[b * cstride + t] : cmake(0.0, 0.0);
+  if (t <= j + 1) sg[t] = needG ? gfix[b * cstride + t] : cmake(0.0, 0.0);
+  __syncthreads();
+  const int ncol = j + 1 > wq ? (j + 1 - wq + kPassWaves - 1) / kPassWaves : 0;
+  c128 corrA[kPassPer], corrG[kPassPer], va[kPassPer], vb[kPassPer];
+#pragma unroll
+  for (int it = 0; it < kPassPer; ++it) {
+    corrA[it] = cmake(0.0, 0.0);
+    corrG[it] = cmake(0.0, 0.0);
+  }
+  auto load = [&](int k, c128(&v)[kPassPer]) {
+    pass_load<kStream, kFull>(Vb + static_cast<long long>(wq + kPassWaves * k) * n + x0, lane, cnt, v);
+  };
+  auto use = [&](int k, const c128(&v)[kPassPer]) {
+    const int c = wq + kPassWaves * k;
+    const c128 a = sa[c], g = sg[c];
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) {
+      corrA[it] = cfma(v[it], a, corrA[it]);
+      corrG[it] = cfma(v[it], g, corrG[it]);
+    }
+  };
+  // the scheduling barriers keep each column's loads issued before the previous column's use (with
+  // few registers in use the compiler would otherwise share va's and vb's registers and wait for
+  // every load before the next column's)
+  if (ncol > 0) load(0, va);
+  int k = 0;
+  for (; k + 2 < ncol; k += 2) {
+    load(k + 1, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    use(k, va);
+    load(k + 2, va);
+    __builtin_amdgcn_sched_barrier(0);
+    use(k + 1, vb);
+  }
+  if (k + 1 < ncol) {
+    load(k + 1, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    use(k, va);
+    use(k + 1, vb);
+  } else if (k < ncol) {
+    use(k, va);
+  }
+  // corrections summed over the waves in a fixed order
+  c128 sA = cmake(0.0, 0.0), sG = cmake(0.0, 0.0);
+  if (needA) {
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) s_corr[wq][it * kWave + lane] = corrA[it];
+    __syncthreads();
+    sA = cadd(cadd(cadd(s_corr[0][t], s_corr[1][t]), s_corr[2][t]), s_corr[3][t]);
+    __syncthreads();
+  }
+  if (needG) {
+#pragma unroll
+    for (int it = 0; it < kPassPer; ++it) s_corr[wq][it * kWave + lane] = corrG[it];
+    __syncthreads();
+    sG = cadd(cadd(cadd(s_corr[0][t], s_corr[1][t]), s_corr[2][t]), s_corr[3][t]);
+  }
+  double ssum = 0.0;
+  if (mine) {
+    if (cm == kCloseRetreat) {
+      const c128 y = csub(cscale(zv, sz), sG);
+      X1[b * n + e] = y;
+      ssum = fma(y.re, y.re, y.im * y.im);
+    } else {
+      const c128 v = csub(cscale(zv, zsc), sA);
+      Vb[static_cast<long long>(j + 1) * n + e] = v;
+      if (cm == kCloseFix) {
+        const c128 vj = Vb[static_cast<long long>(j) * n + e];
+        c128 x = cscale(Q[b * n + e], kq);
+        x = cmake(fma(-kv, vj.re, x.re), fma(-kv, vj.im, x.im));
+        x = cmake(fma(-kz, zv.re, x.re), fma(-kz, zv.im, x.im));
+        x = csub(x, sG);
+        x = cfms(v, sg[j + 1], x);
+        X0[b * n + e] = x;
+        ssum = fma(x.re, x.re, x.im * x.im);
+      }
+    }
+  }
+  if (cm == kCloseTarget) return;
+  ssum = block_sum(ssum, sm);
+  if (t == 0) {
+    fpart[static_cast<long long>(b) * nblk + blk] = ssum;
+    __threadfence();
+    is_last = atomicAdd(&counter[b], 1u) == static_cast<unsigned>(nblk - 1);
+  }
+  __syncthreads();
+  if (!is_last) return;
+  __threadfence();
+  const volatile double* fp = fpart + static_cast<long long>(b) * nblk;
+  double tot = 0.0;
+  for (int i = t; i < nblk; i += kThreads) tot += fp[i];
+  tot = block_sum(tot, sm);
+  const int kk = cm == kCloseRetreat ? j + 1 : j + 2;
+  if (t < kk) cpend[b * cstride + t] = cmake(0.0, 0.0);
+  if (t == 0) {
+    counter[b] = 0;
+    C.nfix += 1;
+    const double bn = std::sqrt(tot);
+    beta[b * astride + kk - 1] = bn;
+    C.npend = 1;
+    C.flags &= ~kCloseMask;
+    if (bn <= tol) {
+      C.active = 0;
+      C.size = kk;
+    } else {
+      C.scale = 1.0 / bn;
+      C.beta_prev = bn;
+      if (cm == kCloseRetreat) {
+        C.stop = j + 1;
+        C.flags |= kBasisShort;
+      }
+    }
+  }
+}
+
 // info of the trajectories that took part: vectors built so far (size is the breakdown size or the
-// eventual target), no-breakdown flag, explicit corrections.
+// eventual target), no-breakdown flag, explicit corrections, and 1 when a paired step stopped the
+// basis early at a near-breakdown (the vectors so far are valid; a later call continues it).
 __global__ void k_basis_info(const BasisCtl* ctl, int* info, int batch) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < batch && ctl[b].part) {
     info[b * 4 + 0] = min(ctl[b].size, ctl[b].stop);
     info[b * 4 + 1] = ctl[b].active;
     info[b * 4 + 2] = ctl[b].nfix;
+    info[b * 4 + 3] = (ctl[b].flags & kBasisShort) ? 1 : 0;
   }
 }
 
@@ -656,12 +1362,13 @@ namespace {
 struct BasisWs {
   c128* dots;        // [batch][units] matvec partials of alpha~_j
   int units;
-  c128* w[2];        // [batch][n] each
-  c128* partial;     // [batch][m][npass] k_lcgs_pass column partials (column j: gamma)
+  c128* w[2];        // [batch][n] each: the pending x_p in w[p % 2]
+  c128* partial;     // [batch][cols][npass] pass column partials (column j: gamma)
   long long pstride;
   int npass;
   double* ypart;     // [batch][npass] |y|^2 partials
-  c128* h;           // [batch][m] reduced column sums
+  c128* h;           // [batch][cols] reduced column sums
+  int hstride;
   c128* cpend;       // [batch][m] pending correction c_{j+1}
   double* fpart;     // [batch][kFixBlocks]
   double* atilde;    // [batch] alpha~_j
@@ -669,12 +1376,21 @@ struct BasisWs {
   unsigned* counter; // [batch]
   double* sq;        // [batch][nchunk]
   BasisCtl* ctl;     // [batch]
+  // paired steps only (null otherwise)
+  c128* z[2];        // [batch][n] each: z / s of the current and the previous pair
+  c128* q;           // [batch][n] H z
+  c128* dots2;       // [batch][units] partials of <z, H z>
+  double* atilde2;   // [batch]
+  c128* cpendA;      // [batch][m] pending correction of the pending z
+  c128* gfix;        // [batch][m + 2] coefficients of an explicit correction
+  double* psc;       // [batch][8] scalars of the pair close
   size_t bytes;
 };
 
-BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
+BasisWs basis_layout(char* base, long long n, int m, int batch, int units, bool paired) {
   const int nchunk = ceil_div(n, kChunk);
   const int npass = ceil_div(n, static_cast<long long>(kPassElems));
+  const int cols = paired ? kPairCols : m;
   size_t off = 0;
   auto take = [&](size_t b) {
     size_t o = off;
@@ -683,19 +1399,36 @@ BasisWs basis_layout(char* base, long long n, int m, int batch, int units) {
   };
   const size_t B = static_cast<size_t>(batch);
   const size_t o_w0 = take(B * n * 16), o_w1 = take(B * n * 16);
-  const size_t o_p = take(B * m * npass * 16), o_yp = take(B * npass * 8), o_h = take(B * m * 16);
+  const size_t o_p = take(B * cols * npass * 16), o_yp = take(B * npass * 8), o_h = take(B * cols * 16);
   const size_t o_c = take(B * m * 16), o_fp = take(B * kFixBlocks * 8), o_cnt = take(B * sizeof(unsigned));
   const size_t o_at = take(B * 8), o_ys = take(B * 8);
   const size_t o_sq = take(B * nchunk * 8), o_ctl = take(B * sizeof(BasisCtl));
   const size_t o_dots = take(B * static_cast<size_t>(units) * 16);
   BasisWs w;
+  if (paired) {
+    const size_t o_z0 = take(B * n * 16), o_z1 = take(B * n * 16), o_q = take(B * n * 16);
+    const size_t o_d2 = take(B * static_cast<size_t>(units) * 16), o_a2 = take(B * 8);
+    const size_t o_ca = take(B * m * 16), o_gf = take(B * (m + 2) * 16), o_ps = take(B * 8 * 8);
+    w.z[0] = reinterpret_cast<c128*>(base + o_z0);
+    w.z[1] = reinterpret_cast<c128*>(base + o_z1);
+    w.q = reinterpret_cast<c128*>(base + o_q);
+    w.dots2 = reinterpret_cast<c128*>(base + o_d2);
+    w.atilde2 = reinterpret_cast<double*>(base + o_a2);
+    w.cpendA = reinterpret_cast<c128*>(base + o_ca);
+    w.gfix = reinterpret_cast<c128*>(base + o_gf);
+    w.psc = reinterpret_cast<double*>(base + o_ps);
+  } else {
+    w.z[0] = w.z[1] = w.q = w.dots2 = w.cpendA = w.gfix = nullptr;
+    w.atilde2 = w.psc = nullptr;
+  }
   w.w[0] = reinterpret_cast<c128*>(base + o_w0);
   w.w[1] = reinterpret_cast<c128*>(base + o_w1);
   w.partial = reinterpret_cast<c128*>(base + o_p);
-  w.pstride = static_cast<long long>(m) * npass;
+  w.pstride = static_cast<long long>(cols) * npass;
   w.npass = npass;
   w.ypart = reinterpret_cast<double*>(base + o_yp);
   w.h = reinterpret_cast<c128*>(base + o_h);
+  w.hstride = cols;
   w.cpend = reinterpret_cast<c128*>(base + o_c);
   w.fpart = reinterpret_cast<double*>(base + o_fp);
   w.atilde = reinterpret_cast<double*>(base + o_at);
@@ -736,17 +1469,21 @@ void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int 
 // Steps j_begin <= j < max_stop of the bases of this call (per trajectory: its window from
 // k_basis_start); without stops, every basis is built whole in one call (j_begin 0, max_stop the
 // target).
+// paired_layout: the workspace holds the buffers of paired steps (the prefix builder's layout, so that
+// calls with and without paired steps can continue each other's bases); paired: steps from
+// kPairFrom on run in pairs.
 int basis_batch(const rpm_operator* op, int batch, const c128* states, long long sld, const int* d_targets,
                 int max_target, const int* d_stops, int j_begin, int max_stop, const c128* pairs, c128* V,
                 long long vstride, int m, double* alpha, double* beta, int* info, double* norm, double tol,
-                void* ws_ptr, size_t ws_bytes, hipStream_t s) {
+                void* ws_ptr, size_t ws_bytes, hipStream_t s, bool paired_layout = false, bool paired = false) {
   const long long n = op->n;
   if (max_target > m || max_target < 1) return fail(RPM_E_INVALID, "target %d outside 1..%d", max_target, m);
   if (max_stop > max_target || max_stop < 1 || j_begin < 0 || j_begin >= max_stop)
     return fail(RPM_E_INVALID, "basis steps %d..%d outside 0..%d", j_begin, max_stop, max_target);
   if (m > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
-  BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch, apply_units(op));
+  BasisWs w = basis_layout(static_cast<char*>(ws_ptr), n, m, batch, apply_units(op), paired_layout);
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "Lanczos workspace %zu < %zu", ws_bytes, w.bytes);
+  if (paired && !paired_layout) return fail(RPM_E_INVALID, "paired Lanczos steps need the prefix workspace");
   const int nchunk = ceil_div(n, kChunk);
   const int full = static_cast<int>(max_target < n ? max_target : n);
   const int count = static_cast<int>(max_stop < n ? max_stop : n);
@@ -755,7 +1492,8 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
   hipLaunchKernelGGL(k_basis_start, dim3(batch), dim3(kThreads), 0, s, w.sq, nchunk, norm, w.ctl, w.counter, d_targets,
                      d_stops, full, n);
   RPM_LAUNCH_CHECK();
-  for (int j = j_begin; j < count; ++j) {
+  const int single_end = paired ? std::min(count, std::max(j_begin, kPairFrom)) : count;
+  for (int j = j_begin; j < single_end; ++j) {
     const c128* in = j == 0 ? states : w.w[j % 2];
     c128* out = w.w[(j + 1) % 2];
     // matvec on x_j = s in: out = H x_j and the per-unit partials of alpha~_j = <x_j, H x_j>; the
@@ -780,14 +1518,91 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     // one pass over V[0..j): v_j materialised, V^H u, gamma, y = u - alpha~ v_j, |y|^2
     launch_pass(w, V, vstride, n, j, in, ild, m, fuse_alpha, out, batch, s);
     RPM_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_cols, dim3(j + 2, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h, m,
-                       nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
+    hipLaunchKernelGGL(k_reduce_cols, dim3(j + 2, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h,
+                       w.hstride, nullptr, m, 0, j, 0, w.ctl, w.ypart, w.ysum);
     RPM_LAUNCH_CHECK();
     // alpha_j, beta_j, c_{j+1}; the explicit correction when it is due
     // the last step of a basis forms alpha only (one block); a prefix's last step also forms beta
     // and the pending correction, so that a later call can resume it
     hipLaunchKernelGGL(k_lcgs_finish, dim3(j == full - 1 ? 1 : kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n,
-                       w.ysum, w.h, m, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter, w.ctl);
+                       w.ysum, w.h, w.hstride, w.cpend, m, w.atilde, alpha, beta, m, j, tol, out, w.fpart, w.counter,
+                       w.ctl);
+    RPM_LAUNCH_CHECK();
+  }
+  // paired slots (j, j + 1), j even; each trajectory takes a pair, a single step or nothing in a slot
+  // from its own window, so its arithmetic does not depend on the other trajectories of the batch
+  // (a slot that starts at an odd j_begin - 1 has nothing live at its first index: every window
+  // of the call begins at j_begin or later)
+  for (int j = single_end & ~1; single_end < count && j < count; j += 2) {
+    c128* zc = w.z[(j / 2) % 2];
+    const c128* zp = w.z[(j / 2 + 1) % 2];
+    ApplyCtl A;
+    A.ctl = w.ctl;
+    A.j = j;
+    A.pairs = pairs;
+    A.dots = w.dots;
+    A.dots_ld = w.units;
+    int rc = operator_apply_ex(op, w.w[0], w.w[1], batch, n, A, s);   // P = H x^_j into w[1]
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, w.atilde, j, w.ctl);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pair_prep, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0, s, V, vstride, n, j,
+                       w.w[0], w.w[1], zp, zc, w.atilde, w.ctl);
+    RPM_LAUNCH_CHECK();
+    A.j = j + 1;
+    A.dots = w.dots2;
+    rc = operator_apply_ex(op, zc, w.q, batch, n, A, s);               // Q = H z (S1: H x^_{j+1})
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots2, w.units, w.atilde2, j + 1, w.ctl);
+    RPM_LAUNCH_CHECK();
+    const int nfull = static_cast<int>(n / kPassElems);
+    if (batch > 1) {
+      if (nfull > 0)
+        hipLaunchKernelGGL((k_pair_pass<true, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j,
+                           w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial, w.pstride,
+                           w.ctl, 0, w.npass);
+      if (nfull < w.npass)
+        hipLaunchKernelGGL((k_pair_pass<true, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
+                           n, j, w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial,
+                           w.pstride, w.ctl, nfull, w.npass);
+    } else {
+      if (nfull > 0)
+        hipLaunchKernelGGL((k_pair_pass<false, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j,
+                           w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial, w.pstride,
+                           w.ctl, 0, w.npass);
+      if (nfull < w.npass)
+        hipLaunchKernelGGL((k_pair_pass<false, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
+                           n, j, w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial,
+                           w.pstride, w.ctl, nfull, w.npass);
+    }
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pair_reduce, dim3(2 * j + 6, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h,
+                       w.hstride, j, w.cpendA, m, w.ctl);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pair_finish, dim3(kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n, w.h, w.hstride,
+                       w.cpendA, w.cpend, w.gfix, m, w.atilde, w.atilde2, alpha, beta, m, j, tol, w.w[0], w.w[1],
+                       w.fpart, w.counter, w.ctl, w.psc);
+    RPM_LAUNCH_CHECK();
+    // (the explicit close's block sums go to ypart, which single steps alone use)
+    if (batch > 1) {
+      if (nfull > 0)
+        hipLaunchKernelGGL((k_pair_close<true, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j, zc,
+                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart, w.counter,
+                           w.ctl, 0, w.npass);
+      if (nfull < w.npass)
+        hipLaunchKernelGGL((k_pair_close<true, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
+                           n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart,
+                           w.counter, w.ctl, nfull, w.npass);
+    } else {
+      if (nfull > 0)
+        hipLaunchKernelGGL((k_pair_close<false, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j, zc,
+                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart, w.counter,
+                           w.ctl, 0, w.npass);
+      if (nfull < w.npass)
+        hipLaunchKernelGGL((k_pair_close<false, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V,
+                           vstride, n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol,
+                           w.ypart, w.counter, w.ctl, nfull, w.npass);
+    }
     RPM_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_basis_info, dim3(ceil_div(batch, 64)), dim3(64), 0, s, w.ctl, info, batch);
@@ -798,12 +1613,17 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
 
 extern "C" size_t rpm_lanczos_workspace_bytes(const rpm_operator* op, int m) {
   if (!op || m < 1) return 0;
-  return basis_layout(nullptr, op->n, m, 1, apply_units(op)).bytes;
+  return basis_layout(nullptr, op->n, m, 1, apply_units(op), false).bytes;
 }
 
 extern "C" size_t rpm_lanczos_batch_workspace_bytes(const rpm_operator* op, int m, int batch) {
   if (!op || m < 1 || batch < 1) return 0;
-  return basis_layout(nullptr, op->n, m, batch, apply_units(op)).bytes;
+  return basis_layout(nullptr, op->n, m, batch, apply_units(op), false).bytes;
+}
+
+extern "C" size_t rpm_lanczos_prefix_workspace_bytes(const rpm_operator* op, int m, int batch) {
+  if (!op || m < 1 || batch < 1) return 0;
+  return basis_layout(nullptr, op->n, m, batch, apply_units(op), true).bytes;
 }
 
 extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, int target, void* d_basis,
@@ -814,7 +1634,7 @@ extern "C" int rpm_lanczos_basis(const rpm_operator* op, const void* d_state, in
   const long long n = op->n;
   const int count = static_cast<int>(target < n ? target : n);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1, apply_units(op));
+  BasisWs w = basis_layout(static_cast<char*>(d_ws), n, count, 1, apply_units(op), false);
   if (ws_bytes < w.bytes) return fail(RPM_E_INVALID, "rpm_lanczos_basis: workspace %zu < %zu", ws_bytes, w.bytes);
   return basis_batch(op, 1, static_cast<const c128*>(d_state), n, nullptr, count, nullptr, 0, count, nullptr,
                      static_cast<c128*>(d_basis), static_cast<long long>(count) * n, count, d_alpha, d_beta, d_info,
@@ -865,7 +1685,7 @@ extern "C" int rpm_lanczos_basis_noalloc(const rpm_operator* op, const void* d_s
   const long long n = op->n;
   const int count = static_cast<int>(target < n ? target : n);
   if (count > kMaxBasis) return fail(RPM_E_UNSUPPORTED, "Krylov dimension above %d is not supported", kMaxBasis);
-  const size_t need = basis_layout(nullptr, n, count, 1, apply_units(op)).bytes;
+  const size_t need = basis_layout(nullptr, n, count, 1, apply_units(op), false).bytes;
   if (need > kNoAllocScratchBytes)
     return fail(RPM_E_INVALID,
                 "rpm_lanczos_basis_noalloc: this basis (N = %lld, %d vectors) needs %zu bytes of scratch, above the "
@@ -899,15 +1719,15 @@ extern "C" int rpm_lanczos_basis_batch_prefix(const rpm_operator* op, int batch,
                                               const int* d_targets, int max_target, const int* d_stops, int j_begin,
                                               int max_stop, const void* d_pairs, void* d_basis, int m,
                                               double* d_alpha, double* d_beta, int* d_info, double* d_norm,
-                                              double tol, void* d_ws, size_t ws_bytes, void* stream) {
+                                              double tol, int flags, void* d_ws, size_t ws_bytes, void* stream) {
   if (!op || !d_states || !d_targets || !d_stops || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || !d_ws ||
-      batch < 1 || batch > 65535 || m < 1)
+      batch < 1 || batch > 65535 || m < 1 || (flags & ~RPM_LANCZOS_PAIRED))
     return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: bad arguments");
   if (ld != op->n) return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: states must be packed (ld == N)");
   return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target, d_stops, j_begin,
                      max_stop, static_cast<const c128*>(d_pairs), static_cast<c128*>(d_basis),
                      static_cast<long long>(m) * op->n, m, d_alpha, d_beta, d_info, d_norm, tol, d_ws, ws_bytes,
-                     static_cast<hipStream_t>(stream));
+                     static_cast<hipStream_t>(stream), true, (flags & RPM_LANCZOS_PAIRED) != 0);
 }
 
 extern "C" int rpm_krylov_combine(const void* d_basis, int k, long long n, const void* d_c, const double* d_scale,

@@ -124,8 +124,12 @@ struct BasisCtl {
   int begin;        // Lanczos steps begin <= j < stop run in the current call (a prefix of the
   int stop;         // basis now, the rest when a later call resumes it)
   int part;         // 1 if the trajectory takes part in the current call
+  int npend;        // paired steps: vectors pending materialisation (1: v_j; 2: v_{j-1} and v_j)
+  double zscale;    // paired steps: v_{j-1} = zscale * z - V c_A for the pending z
+  int flags;        // paired steps: how the pair closes, and kBasisShort
   int pad_;
 };
+constexpr int kBasisShort = 1 << 4;   // the call stopped the basis early; a later call continues it
 
 // Step j of the basis runs for this trajectory in the current call.
 __host__ __device__ inline bool basis_step_live(const BasisCtl& c, int j) {

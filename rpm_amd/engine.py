@@ -664,7 +664,7 @@ class _KrylovWorkspace:
         self.beta = torch.zeros(m, dtype=torch.float64, device=dev)
         self.info = torch.zeros(4, dtype=torch.int32, device=dev)
         self.norm = torch.zeros(1, dtype=torch.float64, device=dev)
-        self.ws = nat.workspace(lib.rpm_lanczos_workspace_bytes(ctypes.c_void_p(op.handle), m))
+        self.ws = nat.workspace(lib.rpm_lanczos_prefix_workspace_bytes(ctypes.c_void_p(op.handle), m, 1))
         self.err = torch.zeros(16, dtype=torch.float64, device=dev)
         self.coeffs = torch.zeros((16, m), dtype=torch.complex128, device=dev)
         self.scratch = nat.workspace(lib.rpm_krylov_scratch_bytes(16, m))
@@ -705,6 +705,8 @@ class AdaptiveLanczosPropagator:
         self._built = 0
         # replay each basis step window as a HIP graph (captured once per propagation)
         self.use_graphs = False
+        # Lanczos steps from the 12th on in pairs that share one read of the basis
+        self.paired_steps = True
         self._graphs = None
         # matvecs the device made in the last propagate call, and the vectors of each attempt's basis;
         # stats.matvecs (and the Hamiltonian's matvec_count) count the reference's whole-basis calls
@@ -739,20 +741,29 @@ class AdaptiveLanczosPropagator:
         j0 = self._built if resume else 0
         graphs = self._graphs if (self._graphs is not None and self._graphs.owner is ws) else None
 
-        def launch(stream):
+        flags = nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0
+
+        def launch(stream, j0=j0):
             nat.check(lib.rpm_lanczos_basis_batch_prefix(
                 ctypes.c_void_p(op.handle), 1, ctypes.c_void_p(nat.ptr(state)), n,
                 ctypes.c_void_p(nat.ptr(ws.targets)), ws.m, ctypes.c_void_p(nat.ptr(ws.targets) + 4), j0, int(ts[1]),
                 None, ctypes.c_void_p(nat.ptr(ws.V)), ws.m, ctypes.c_void_p(nat.ptr(ws.alpha)),
                 ctypes.c_void_p(nat.ptr(ws.beta)), ctypes.c_void_p(nat.ptr(ws.info)), ctypes.c_void_p(nat.ptr(ws.norm)),
-                float(self.breakdown_tolerance), ctypes.c_void_p(nat.ptr(ws.ws)), ws.ws.numel(),
+                float(self.breakdown_tolerance), flags, ctypes.c_void_p(nat.ptr(ws.ws)), ws.ws.numel(),
                 ctypes.c_void_p(stream)))
 
         if graphs is not None:
             graphs.run((nat.ptr(state), j0, int(ts[1])), launch)
         else:
             launch(nat.stream_ptr())
-        size = int(ws.info[0].item())
+        info = ws.info.cpu().numpy()
+        size = int(info[0])
+        while info[3]:
+            # a paired step stopped the basis early at a near-breakdown: continue it to the stop
+            ws.targets.copy_(torch.from_numpy(np.array([0, ts[1]], dtype=np.int32)))
+            launch(nat.stream_ptr(), j0=size)
+            info = ws.info.cpu().numpy()
+            size = int(info[0])
         self.device_matvecs += size - j0
         self._built = size
         return size

@@ -88,6 +88,7 @@ def test_batched_basis_matches_single_basis():
     assert info_h[2, 0] == -1 and norm[2].item() == -1.0, "a trajectory with target 0 must stay untouched"
     assert np.all(alpha[2].cpu().numpy() == -7.0)
     prop = engine.AdaptiveLanczosPropagator(h, minimum_dimension=8, maximum_dimension=24)
+    prop.paired_steps = False       # the whole-basis entry point takes one step per read of the basis
     for b, t in enumerate(targets):
         if t == 0:
             continue
