@@ -272,7 +272,7 @@ int rpm_axis_apply(int naxes, const int* dims, int axis, const void* d_U, const 
  * Adjoint sweep of the Simpson singlet yield Phi = sum_k W_k <psi_k|P|psi_k> on an output grid
  * (no reference counterpart; the reference computes Phi only, R/rpm_quantum_needle_jd_sweeps.py:418-420).
  * The grid is cut into nmacro steps [t_a, t_b]; step k has class cls[k] (series length
- * nterms[c] = K <= kmax, coefficients coef[c][0..K] (host, interleaved complex) of
+ * nterms[c] = K <= kmax <= 255, coefficients coef[c][0..K] (host, interleaved complex) of
  * exp(-i H (t_b - t_a)) = sum_n c_n T_n(X), X = (H - shift)/half_width, and the (kmax+1)^2 complex
  * matrix d_G[c] (device) with G[m][j] = sum_{kicks t in (t_a, t_b]} 2 w(t) conj(c_m(t - t_a)) c_j(t - t_a)),
  * a weight multiplier mult[k] (W = mult[k] w), electron-matrix segment seg[k] < nseg (d_pairs: nseg x batch

@@ -23,8 +23,11 @@ represented by their Chebyshev expansions (Tal-Ezer & Kosloff 1984) to
   the Clenshaw vectors of the costate (the discrete adjoint of the Chebyshev
   recurrence), accumulated as electron cross-densities.
 
-Per grid interval this costs about 3K matvecs (K ~ 16 at the 0.00125 us
-grid), independent of the number of parameters.
+The sweep takes steps of up to MACRO_INTERVALS grid intervals, one Chebyshev
+series per step, with the output-time kicks of the step folded into the
+Clenshaw seeds. Per grid interval this costs about 3K/64 matvecs (K = 122 for
+64 intervals of 0.00125 us on the production operator), independent of the
+number of parameters.
 """
 
 from __future__ import annotations
@@ -115,7 +118,12 @@ def spectral_bounds(hamiltonian, pairs=None):
     return lo + float(ev.min()), hi + float(ev.max())
 
 
-MACRO_INTERVALS = 16      # output-grid intervals per backward Chebyshev step
+# Output-grid intervals per backward Chebyshev step. Longer steps need fewer matvecs per interval
+# (K grows more slowly than the step): on 8 production trajectories over 2 us the sweep took 1.05,
+# 0.76, 0.67, 0.60, 0.55 and 0.52 s at 8, 16, 24, 32, 48 and 64 intervals per step, with gradients
+# equal to 1e-13. A step whose series would pass MAX_TERMS is halved until it fits.
+MACRO_INTERVALS = 64
+MAX_TERMS = 255           # Chebyshev terms the device tables hold
 
 
 def chebyshev_series_fixed(h: float, shift: float, half_width: float, K: int) -> np.ndarray:
@@ -232,11 +240,16 @@ def adjoint_densities(hamiltonian, final_states, times_us, weights, *, pairs=Non
     lo, hi = spectral_bounds(h, p)
     shift = 0.5 * (lo + hi)
     half = 0.5 * (hi - lo) * (1.0 + 1e-9) + 1e-12
-    steps = _macro_steps(M, seg, slot, max(1, int(intervals_per_step)))
-    classes, cls, mult = _step_classes(times, w, steps, shift, half, decay_time)
-    kmax = max(c.K for c in classes)
-    if kmax > 127:
-        raise ValueError("sweep steps too long for the Chebyshev tables (use fewer intervals per step)")
+    per = max(1, int(intervals_per_step))
+    while True:
+        steps = _macro_steps(M, seg, slot, per)
+        classes, cls, mult = _step_classes(times, w, steps, shift, half, decay_time)
+        kmax = max(c.K for c in classes)
+        if kmax <= MAX_TERMS or per == 1:
+            break
+        per = max(1, per // 2)
+    if kmax > MAX_TERMS:
+        raise ValueError("sweep steps too long for the Chebyshev tables (grid intervals too long for the spectrum)")
     coef = np.zeros((len(classes), kmax + 1), dtype=complex)
     Gs = np.zeros((len(classes), kmax + 1, kmax + 1), dtype=complex)
     nterms = np.zeros(len(classes), dtype=np.int32)

@@ -163,13 +163,14 @@ __global__ void __launch_bounds__(256) k_singlet_diff(const c128* __restrict__ u
 //   S_m = sum_j G[m][j] chi_j,  m in [blockIdx.z * kSeedRows, ...) and m < nm,
 // summed over j in ascending order for every m.
 constexpr int kSeedRows = 8;
+constexpr int kMaxTerms = 255;     // Chebyshev terms per sweep step: K + 1 <= 256 columns of G in LDS
 __global__ void __launch_bounds__(256) k_seed_sums(const c128* __restrict__ chi, long long cstride, long long cld,
                                                    int nj, const c128* __restrict__ G, int gstride, int nm,
                                                    c128* __restrict__ S, long long nn) {
   const long long b = blockIdx.y;
   const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   const int m0 = static_cast<int>(blockIdx.z) * kSeedRows;
-  __shared__ c128 gs[kSeedRows][128];
+  __shared__ c128 gs[kSeedRows][kMaxTerms + 1];
   for (int t = threadIdx.x; t < kSeedRows * nj; t += 256) {
     const int i = t / nj, j = t % nj;
     gs[i][j] = m0 + i < nm ? G[static_cast<long long>(m0 + i) * gstride + j] : cmake(0.0, 0.0);
@@ -281,7 +282,7 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
                                  const int* seg, int nseg, void* d_acc, const int* slot, int nslot, void* d_ws,
                                  size_t ws_bytes, void* stream) {
   if (!op || op->kind != 1 || !d_psi || nmacro < 1 || !nterms || !coef || !d_G || !cls || !mult || !d_acc ||
-      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || kmax > 127 || !(half_width > 0.0) ||
+      !slot || !d_ws || batch < 1 || batch > 65535 || ncls < 1 || kmax < 2 || kmax > kMaxTerms || !(half_width > 0.0) ||
       nslot < 1)
     return fail(RPM_E_INVALID, "rpm_adjoint_sweep: bad arguments");
   if (d_pairs && (!seg || nseg < 1))
