@@ -196,8 +196,12 @@ int rpm_lanczos_basis_batch(const rpm_operator* op, int batch, const void* d_sta
  * bases.  A near-breakdown inside a pair stops that trajectory's basis early with info[4b+3] = 1 and
  * info[4b+1] = 1: the vectors so far are valid, and a further call (d_targets[b] = 0) continues it.
  * The result is deterministic per trajectory (independent of the other trajectories of the batch).
+ * flags |= RPM_LANCZOS_FORCE_SHORT (with RPM_LANCZOS_PAIRED; a test hook): every pair takes that
+ * early stop at its first step, so the continuation path can be exercised; the basis built over the
+ * continuing calls is the same to round-off.
  */
 #define RPM_LANCZOS_PAIRED 1
+#define RPM_LANCZOS_FORCE_SHORT 2
 size_t rpm_lanczos_prefix_workspace_bytes(const rpm_operator* op, int m, int batch);
 int rpm_lanczos_basis_batch_prefix(const rpm_operator* op, int batch, const void* d_states, long long ld,
                                    const int* d_targets, int max_target, const int* d_stops, int j_begin,

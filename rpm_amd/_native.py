@@ -21,6 +21,7 @@ RPM_E_HIP = -2
 RPM_E_NOMEM = -3
 RPM_E_UNSUPPORTED = -4
 RPM_LANCZOS_PAIRED = 1          # rpm_lanczos_basis_batch_prefix: two steps per read of the basis
+RPM_LANCZOS_FORCE_SHORT = 2     # test hook: every pair stops its call early, as at a near-breakdown
 
 _c_int_p = ctypes.POINTER(ctypes.c_int)
 _c_double_p = ctypes.POINTER(ctypes.c_double)

@@ -135,6 +135,9 @@ class BatchedLanczosPropagator:
         self.graphs = None
         # Lanczos steps from the 12th on in pairs that share one read of the basis
         self.paired_steps = True
+        # extra flags of the basis builder for tests (RPM_LANCZOS_FORCE_SHORT exercises the
+        # continuation of a basis that a call stopped early)
+        self._basis_test_flags = 0
 
     @classmethod
     def like(cls, propagator, hamiltonian=None) -> "BatchedLanczosPropagator":
@@ -366,7 +369,7 @@ class BatchedLanczosPropagator:
         info = torch.zeros((B, 4), dtype=torch.int32, device=dev)
         norm = torch.zeros(B, dtype=torch.float64, device=dev)
         ws = nat.workspace(lib.rpm_lanczos_prefix_workspace_bytes(ctypes.c_void_p(op.handle), m, B))
-        flags = nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0
+        flags = (nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0) | self._basis_test_flags
         gram_ws = nat.workspace(lib.rpm_singlet_gram_batch_workspace_bytes(n // 4, B))
         P = torch.zeros((B, m, m), dtype=torch.complex128, device=dev)
         ladder_len = len(self._ladder(self.minimum_dimension))

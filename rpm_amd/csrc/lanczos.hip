@@ -887,7 +887,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_finish(const c128* __restrict
                                                           double* __restrict__ beta, int astride, int j, double tol,
                                                           c128* __restrict__ X0, c128* __restrict__ X1,
                                                           double* __restrict__ fpart, unsigned* __restrict__ counter,
-                                                          BasisCtl* ctl, double* __restrict__ psc) {
+                                                          BasisCtl* ctl, double* __restrict__ psc, int force_short) {
   const int b = blockIdx.y;
   BasisCtl& C = ctl[b];
   const int mode = pair_mode(C, j);
@@ -941,8 +941,8 @@ __global__ void __launch_bounds__(kThreads) k_pair_finish(const c128* __restrict
   const double aj = gam.re - bp * cprev;
   if (t == 0) alpha[b * astride + j] = aj;
   const double b2 = yy - gg;
-  if (!(b2 > kCancel * yy)) {
-    // near-breakdown at step j: k_pair_close corrects z and stops the basis at j + 1
+  if (!(b2 > kCancel * yy) || force_short) {
+    // near-breakdown at step j (or the test hook): k_pair_close corrects z and stops the basis at j + 1
     if (t < j) gf[t] = graw;
     if (t == 0) {
       gf[j] = glast;
@@ -1471,11 +1471,13 @@ void launch_pass(const BasisWs& w, c128* V, long long vstride, long long n, int 
 // target).
 // paired_layout: the workspace holds the buffers of paired steps (the prefix builder's layout, so that
 // calls with and without paired steps can continue each other's bases); paired: steps from
-// kPairFrom on run in pairs.
+// kPairFrom on run in pairs; force_short (a test hook): every pair stops its trajectory's call as a
+// near-breakdown at its first step would.
 int basis_batch(const rpm_operator* op, int batch, const c128* states, long long sld, const int* d_targets,
                 int max_target, const int* d_stops, int j_begin, int max_stop, const c128* pairs, c128* V,
                 long long vstride, int m, double* alpha, double* beta, int* info, double* norm, double tol,
-                void* ws_ptr, size_t ws_bytes, hipStream_t s, bool paired_layout = false, bool paired = false) {
+                void* ws_ptr, size_t ws_bytes, hipStream_t s, bool paired_layout = false, bool paired = false,
+                bool force_short = false) {
   const long long n = op->n;
   if (max_target > m || max_target < 1) return fail(RPM_E_INVALID, "target %d outside 1..%d", max_target, m);
   if (max_stop > max_target || max_stop < 1 || j_begin < 0 || j_begin >= max_stop)
@@ -1581,7 +1583,7 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pair_finish, dim3(kFixBlocks, batch), dim3(kThreads), 0, s, V, vstride, n, w.h, w.hstride,
                        w.cpendA, w.cpend, w.gfix, m, w.atilde, w.atilde2, alpha, beta, m, j, tol, w.w[0], w.w[1],
-                       w.fpart, w.counter, w.ctl, w.psc);
+                       w.fpart, w.counter, w.ctl, w.psc, force_short ? 1 : 0);
     RPM_LAUNCH_CHECK();
     // (the explicit close's block sums go to ypart, which single steps alone use)
     if (batch > 1) {
@@ -1721,13 +1723,14 @@ extern "C" int rpm_lanczos_basis_batch_prefix(const rpm_operator* op, int batch,
                                               double* d_alpha, double* d_beta, int* d_info, double* d_norm,
                                               double tol, int flags, void* d_ws, size_t ws_bytes, void* stream) {
   if (!op || !d_states || !d_targets || !d_stops || !d_basis || !d_alpha || !d_beta || !d_info || !d_norm || !d_ws ||
-      batch < 1 || batch > 65535 || m < 1 || (flags & ~RPM_LANCZOS_PAIRED))
+      batch < 1 || batch > 65535 || m < 1 || (flags & ~(RPM_LANCZOS_PAIRED | RPM_LANCZOS_FORCE_SHORT)))
     return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: bad arguments");
   if (ld != op->n) return fail(RPM_E_INVALID, "rpm_lanczos_basis_batch_prefix: states must be packed (ld == N)");
   return basis_batch(op, batch, static_cast<const c128*>(d_states), ld, d_targets, max_target, d_stops, j_begin,
                      max_stop, static_cast<const c128*>(d_pairs), static_cast<c128*>(d_basis),
                      static_cast<long long>(m) * op->n, m, d_alpha, d_beta, d_info, d_norm, tol, d_ws, ws_bytes,
-                     static_cast<hipStream_t>(stream), true, (flags & RPM_LANCZOS_PAIRED) != 0);
+                     static_cast<hipStream_t>(stream), true, (flags & RPM_LANCZOS_PAIRED) != 0,
+                     (flags & RPM_LANCZOS_FORCE_SHORT) != 0);
 }
 
 extern "C" int rpm_krylov_combine(const void* d_basis, int k, long long n, const void* d_c, const double* d_scale,

@@ -707,6 +707,9 @@ class AdaptiveLanczosPropagator:
         self.use_graphs = False
         # Lanczos steps from the 12th on in pairs that share one read of the basis
         self.paired_steps = True
+        # extra flags of the basis builder for tests (RPM_LANCZOS_FORCE_SHORT exercises the
+        # continuation of a basis that a call stopped early)
+        self._basis_test_flags = 0
         self._graphs = None
         # matvecs the device made in the last propagate call, and the vectors of each attempt's basis;
         # stats.matvecs (and the Hamiltonian's matvec_count) count the reference's whole-basis calls
@@ -741,7 +744,7 @@ class AdaptiveLanczosPropagator:
         j0 = self._built if resume else 0
         graphs = self._graphs if (self._graphs is not None and self._graphs.owner is ws) else None
 
-        flags = nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0
+        flags = (nat.RPM_LANCZOS_PAIRED if self.paired_steps else 0) | self._basis_test_flags
 
         def launch(stream, j0=j0):
             nat.check(lib.rpm_lanczos_basis_batch_prefix(

@@ -151,6 +151,60 @@ def test_paired_basis_of_a_trajectory_does_not_depend_on_the_batch():
             assert torch.equal(one.buf[key][0], full.buf[key][b]), (b, key)
 
 
+def test_forced_early_stops_continue_to_the_same_basis():
+    """RPM_LANCZOS_FORCE_SHORT makes every pair stop its trajectory's call at its first step, as a
+    near-breakdown inside a pair does; calls continued until no trajectory is short give the
+    one-call basis to round-off."""
+    print("case: production operator, 3 trajectories, every pair stopped early, continued", flush=True)
+    h = core.build_hamiltonian(90.0, 0.05)
+    thetas = (0.0, 45.0, 90.0)
+    ref = _Prefix(h, thetas, range(3))
+    ref.call([64] * 3, [64] * 3, 0, flags=0)
+    got = _Prefix(h, thetas, range(3))
+    flags = nat.RPM_LANCZOS_PAIRED | nat.RPM_LANCZOS_FORCE_SHORT
+    info = got.call([64, 64, 64], [64, 40, 64], 0, flags=flags)
+    calls, shorts = 1, int(info[:, 3].sum())
+    while info[:, 3].any():
+        stops = [64 if info[b, 3] and b != 1 else (40 if info[b, 3] else 0) for b in range(3)]
+        j0 = int(min(info[b, 0] for b in range(3) if info[b, 3]))
+        info = got.call([0, 0, 0], stops, j0, flags=flags)
+        calls += 1
+        shorts += int(info[:, 3].sum())
+    info = got.call([0, 0, 0], [0, 64, 0], 40, flags=flags)       # trajectory 1 continues past its prefix
+    while info[:, 3].any():
+        info = got.call([0, 0, 0], [0, 64, 0], int(info[1, 0]), flags=flags)
+        calls += 1
+    print("calls", calls, "early stops", shorts, flush=True)
+    assert shorts >= 10 and list(got.buf["info"][:, 0].cpu().numpy()) == [64] * 3
+    for b in range(3):
+        scale = ref.buf["alpha"][b].abs().max().item()
+        assert (got.buf["V"][b] - ref.buf["V"][b]).abs().max().item() < 1e-12
+        assert (got.buf["alpha"][b] - ref.buf["alpha"][b]).abs().max().item() < 1e-12 * scale
+        assert (got.buf["beta"][b, :63] - ref.buf["beta"][b, :63]).abs().max().item() < 1e-12 * scale
+
+
+def test_propagators_continue_bases_stopped_early():
+    """Both propagators continue a basis a call stopped early (forced at every pair) and keep the
+    schedule and values of the default run."""
+    print("case: production operator, 0.1 us, single and batched, every pair stopped early", flush=True)
+    h = core.build_hamiltonian(90.0, 0.05)
+    times = np.arange(81) * 0.00125
+    states = np.stack([core.random_phase_initial_state(h.dimensions, core.MASTER_SEED, i) for i in range(2)])
+    kw = dict(state_error_budget=2.4e-5 * 0.1 / 57.56)
+    obs = engine.SingletPopulationObservable(h.dimensions)
+    ref = engine.AdaptiveLanczosPropagator(h, **kw).propagate(states[0], times, obs)
+    single = engine.AdaptiveLanczosPropagator(h, **kw)
+    single._basis_test_flags = nat.RPM_LANCZOS_FORCE_SHORT
+    got = single.propagate(states[0], times, obs)
+    assert got.schedule == ref.schedule
+    np.testing.assert_allclose(got.observable_values, ref.observable_values, rtol=0, atol=1e-12)
+    bp = batch.BatchedLanczosPropagator(h, **kw)
+    bp._basis_test_flags = nat.RPM_LANCZOS_FORCE_SHORT
+    res = bp.propagate(states, times)
+    assert res[0].schedule == ref.schedule
+    np.testing.assert_allclose(res[0].observable_values, ref.observable_values, rtol=0, atol=1e-12)
+
+
 def test_propagators_build_fewer_vectors_with_the_same_schedule():
     print("case: production operator, 0.3 us, single and batched", flush=True)
     h = core.build_hamiltonian(90.0, 0.05)

@@ -73,8 +73,10 @@ def test_last_error_and_invalid_arguments_without_gpu():
     assert lib.rpm_lanczos_prefix_workspace_bytes(None, 8, 1) == 0
     # unknown flag bits are rejected before the operator is touched
     one = ctypes.c_void_p(1)
+    bad = 1 << 5
+    assert not bad & (_native.RPM_LANCZOS_PAIRED | _native.RPM_LANCZOS_FORCE_SHORT)
     rc = lib.rpm_lanczos_basis_batch_prefix(one, 1, one, 4, one, 8, one, 0, 8, None, one, 8, one, one, one, one, 1e-14,
-                                            2, one, 1 << 20, None)
+                                            bad, one, 1 << 20, None)
     assert rc == _native.RPM_E_INVALID
 
 
