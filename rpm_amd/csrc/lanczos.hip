@@ -1019,20 +1019,17 @@ __global__ void __launch_bounds__(kThreads) k_pair_finish(const c128* __restrict
 
 // Vector work of a pair after k_pair_finish; single steps of the slot leave one vector pending.
 // Grid (blocks of kPassElems elements, batch), in one launch, or in two (kFull, then the last
-// partial block) that run in stream order: the last block of a trajectory to finish (device-scope
-// counter over all npass blocks) takes the explicit norm. Three-term close: x_{j+2} elementwise.
-// Explicit closes stream V_<=j as the pass does (waves over columns, the next column's loads in
-// flight) for the corrections V_<=j c_A' (v_{j+1}) and V_<=j g (x_{j+2} or, for a retreat, z).
+// partial block). Three-term close: x_{j+2} elementwise. Explicit closes stream V_<=j as the pass
+// does (waves over columns, the next column's loads in flight) for the corrections V_<=j c_A'
+// (v_{j+1}) and V_<=j g (x_{j+2} or, for a retreat, z), and leave their blocks' |x|^2 for k_pair_norm.
 template <bool kStream, bool kFull>
 __global__ void __launch_bounds__(kThreads) k_pair_close(c128* __restrict__ V, long long vstride, long long n, int j,
                                                          const c128* __restrict__ Z, const c128* __restrict__ Q,
                                                          c128* __restrict__ X0, c128* __restrict__ X1,
                                                          const c128* __restrict__ cpendA, c128* __restrict__ cpend,
                                                          const c128* __restrict__ gfix, int cstride,
-                                                         const double* __restrict__ psc, double* __restrict__ beta,
-                                                         int astride, double tol, double* __restrict__ fpart,
-                                                         unsigned* __restrict__ counter, BasisCtl* ctl, int blk0,
-                                                         int nblk) {
+                                                         const double* __restrict__ psc, double* __restrict__ fpart,
+                                                         BasisCtl* ctl, int blk0, int nblk) {
   const int b = blockIdx.y;
   BasisCtl& C = ctl[b];
   const int mode = pair_mode(C, j);
@@ -1070,7 +1067,6 @@ __global__ void __launch_bounds__(kThreads) k_pair_close(c128* __restrict__ V, l
   __shared__ c128 sg[kMaxBasis + 2];
   __shared__ c128 s_corr[kPassWaves][kPassElems];
   __shared__ double sm[kThreads / kWave];
-  __shared__ int is_last;
   const bool needA = cm != kCloseRetreat, needG = cm != kCloseTarget;
   // (a retreat has no c_A', a target close no g: their products are computed and not used)
   if (t <= j) sa[t] = needA ? cpendA[b * cstride + t] : cmake(0.0, 0.0);
@@ -1154,22 +1150,32 @@ __global__ void __launch_bounds__(kThreads) k_pair_close(c128* __restrict__ V, l
   }
   if (cm == kCloseTarget) return;
   ssum = block_sum(ssum, sm);
-  if (t == 0) {
-    fpart[static_cast<long long>(b) * nblk + blk] = ssum;
-    __threadfence();
-    is_last = atomicAdd(&counter[b], 1u) == static_cast<unsigned>(nblk - 1);
-  }
-  __syncthreads();
-  if (!is_last) return;
-  __threadfence();
-  const volatile double* fp = fpart + static_cast<long long>(b) * nblk;
+  if (t == 0) fpart[static_cast<long long>(b) * nblk + blk] = ssum;
+}
+
+// The norm of an explicit close (k_pair_close's block sums of |x|^2, or |y|^2 for a retreat, summed in
+// a fixed order) and the control-block update that goes with it. A kernel of its own: a last-block
+// reduction inside k_pair_close would put a device-scope fence (a write-back of the whole L2 on this
+// chip) in each of its blocks, which made the close of a prefix flush take 2.5 ms at 32 trajectories.
+// Grid (batch), one block per trajectory.
+__global__ void __launch_bounds__(kThreads) k_pair_norm(const double* __restrict__ fpart, int nblk, int j,
+                                                        c128* __restrict__ cpend, int cstride,
+                                                        double* __restrict__ beta, int astride, double tol,
+                                                        BasisCtl* ctl) {
+  const int b = blockIdx.x;
+  BasisCtl& C = ctl[b];
+  if (pair_mode(C, j) != kModePair) return;
+  const int cm = C.flags & kCloseMask;
+  if (cm != kCloseFix && cm != kCloseRetreat) return;
+  __shared__ double sm[kThreads / kWave];
+  const int t = threadIdx.x;
+  const double* fp = fpart + static_cast<long long>(b) * nblk;
   double tot = 0.0;
   for (int i = t; i < nblk; i += kThreads) tot += fp[i];
-  tot = block_sum(tot, sm);
+  tot = block_sum(tot, sm);   // ends with a barrier: every thread has read C before it is written
   const int kk = cm == kCloseRetreat ? j + 1 : j + 2;
   if (t < kk) cpend[b * cstride + t] = cmake(0.0, 0.0);
   if (t == 0) {
-    counter[b] = 0;
     C.nfix += 1;
     const double bn = std::sqrt(tot);
     beta[b * astride + kk - 1] = bn;
@@ -1589,22 +1595,23 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     if (batch > 1) {
       if (nfull > 0)
         hipLaunchKernelGGL((k_pair_close<true, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j, zc,
-                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart, w.counter,
-                           w.ctl, 0, w.npass);
+                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, w.ypart, w.ctl, 0, w.npass);
       if (nfull < w.npass)
         hipLaunchKernelGGL((k_pair_close<true, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
-                           n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart,
-                           w.counter, w.ctl, nfull, w.npass);
+                           n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, w.ypart, w.ctl, nfull,
+                           w.npass);
     } else {
       if (nfull > 0)
         hipLaunchKernelGGL((k_pair_close<false, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j, zc,
-                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol, w.ypart, w.counter,
-                           w.ctl, 0, w.npass);
+                           w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, w.ypart, w.ctl, 0, w.npass);
       if (nfull < w.npass)
         hipLaunchKernelGGL((k_pair_close<false, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V,
-                           vstride, n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, beta, m, tol,
-                           w.ypart, w.counter, w.ctl, nfull, w.npass);
+                           vstride, n, j, zc, w.q, w.w[0], w.w[1], w.cpendA, w.cpend, w.gfix, m, w.psc, w.ypart, w.ctl,
+                           nfull, w.npass);
     }
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_pair_norm, dim3(batch), dim3(kThreads), 0, s, w.ypart, w.npass, j, w.cpend, m, beta, m, tol,
+                       w.ctl);
     RPM_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(k_basis_info, dim3(ceil_div(batch, 64)), dim3(64), 0, s, w.ctl, info, batch);
