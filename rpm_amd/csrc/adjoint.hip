@@ -20,9 +20,12 @@ __global__ void __launch_bounds__(256) k_zero(c128* __restrict__ x, long long n)
 constexpr int kDensThreads = 256;
 constexpr int kDensPer = 8;   // nuclear indices per thread
 
-// partial[(b * nblk + blk) * 16 + e*4 + e'] = sum over the block's nuclear indices of conj(a_e) b_e'
+// partial[(b * nblk + blk) * 16 + e*4 + e'] = weight * (sum over the block's nuclear indices of
+// conj(a_e) b_e'), added to the partial already there when `accumulate` (the sweep sums all
+// Chebyshev orders of a step in the block partials and reduces them once per step)
 __global__ void __launch_bounds__(kDensThreads) k_cross_density(const c128* __restrict__ A, const c128* __restrict__ Bv,
-                                                                long long ld, long long nn, c128* __restrict__ partial) {
+                                                                long long ld, long long nn, c128* __restrict__ partial,
+                                                                double weight, int accumulate) {
   __shared__ c128 sm[kDensThreads / kWave];
   const long long b = blockIdx.y;
   const c128* a = A + b * ld;
@@ -49,7 +52,12 @@ __global__ void __launch_bounds__(kDensThreads) k_cross_density(const c128* __re
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const c128 t = block_sum(acc[i], sm);
-    if (threadIdx.x == 0) out[i] = t;
+    if (threadIdx.x == 0) {
+      c128 v = accumulate ? out[i] : cmake(0.0, 0.0);
+      v.re = fma(weight, t.re, v.re);
+      v.im = fma(weight, t.im, v.im);
+      out[i] = v;
+    }
   }
 }
 
@@ -116,7 +124,7 @@ extern "C" int rpm_cross_density(const void* d_a, const void* d_b, int batch, lo
   const int nblk = ceil_div(nn, kDensThreads * kDensPer);
   c128* part = static_cast<c128*>(d_ws);
   hipLaunchKernelGGL(k_cross_density, dim3(nblk, batch), dim3(kDensThreads), 0, s, static_cast<const c128*>(d_a),
-                     static_cast<const c128*>(d_b), ld, nn, part);
+                     static_cast<const c128*>(d_b), ld, nn, part, 1.0, 0);
   RPM_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_density_finish, dim3(batch), dim3(64), 0, s, part, nblk, weight, static_cast<c128*>(d_acc));
   RPM_LAUNCH_CHECK();
@@ -149,13 +157,16 @@ extern "C" int rpm_axis_apply(int naxes, const int* dims, int axis, const void* 
 namespace rpm {
 
 // chi[x] = u[e=1][x] - u[e=2][x] for x < nn (the singlet part of u, times sqrt2).
+// chi of vector set z (grid.z) at u + z * ustride, into chi + z * cstride
 __global__ void __launch_bounds__(256) k_singlet_diff(const c128* __restrict__ u, c128* __restrict__ chi, long long ld,
-                                                      long long cld, long long nn) {
+                                                      long long cld, long long nn, long long ustride,
+                                                      long long cstride) {
   const long long b = blockIdx.y;
   const long long x = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
   if (x >= nn) return;
-  const c128 p = u[b * ld + nn + x], q = u[b * ld + 2 * nn + x];
-  chi[b * cld + x] = csub(p, q);
+  const c128* uz = u + blockIdx.z * ustride;
+  const c128 p = uz[b * ld + nn + x], q = uz[b * ld + 2 * nn + x];
+  chi[blockIdx.z * cstride + b * cld + x] = csub(p, q);
 }
 
 // Singlet sums of the costate seeds of every Chebyshev order of a step, in one pass over the
@@ -243,8 +254,21 @@ int scaled_copy(const SweepCtx& C, c128 alpha, const c128* x, c128* out) {
   return RPM_OK;
 }
 
-int density(const SweepCtx& C, const c128* a, const c128* b, double weight, c128* acc) {
-  return rpm_cross_density(a, b, C.batch, C.n, C.nn, weight, acc, C.dens_ws, C.dens_bytes, C.s);
+// weight * C(a, b) added to the sweep step's density block partials (started by `first`); the
+// step's partials are reduced into its accumulator once, by density_finish
+int density(const SweepCtx& C, const c128* a, const c128* b, double weight, bool first) {
+  const int nblk = ceil_div(C.nn, kDensThreads * kDensPer);
+  hipLaunchKernelGGL(k_cross_density, dim3(nblk, C.batch), dim3(kDensThreads), 0, C.s, a, b, C.n, C.nn, C.dens_ws,
+                     weight, first ? 0 : 1);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+int density_finish(const SweepCtx& C, c128* acc) {
+  const int nblk = ceil_div(C.nn, kDensThreads * kDensPer);
+  hipLaunchKernelGGL(k_density_finish, dim3(C.batch), dim3(64), 0, C.s, C.dens_ws, nblk, 1.0, acc);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
 }
 
 inline c128 conjc(const double* c) { return cmake(c[0], -c[1]); }
@@ -351,11 +375,12 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
       return rc;
     for (int n = 1; n < K; ++n)
       if ((rc = affine(C, u[n], u[n + 1], pairs, s2, a2, nullptr, cmake(0.0, 0.0), u[n - 1], -1.0))) return rc;
-    for (int j = 0; j <= K; ++j) {
-      hipLaunchKernelGGL(k_singlet_diff, dim3(nqb, batch), dim3(256), 0, C.s, u[j], chi + j * cstride, C.n, C.nn,
-                         C.nn);
-      RPM_LAUNCH_CHECK();
-    }
+    // chi_0 from psi_a, chi_1..chi_K from the stored T_j(X) psi_a (evenly spaced in the workspace)
+    hipLaunchKernelGGL(k_singlet_diff, dim3(nqb, batch, 1), dim3(256), 0, C.s, u[0], chi, C.n, C.nn, C.nn, 0LL, 0LL);
+    RPM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_singlet_diff, dim3(nqb, batch, K), dim3(256), 0, C.s, u[1], chi + cstride, C.n, C.nn, C.nn,
+                       static_cast<long long>(vec / sizeof(c128)), cstride);
+    RPM_LAUNCH_CHECK();
     // Clenshaw over the costate with vector seeds z_m = conj(c_m) g + mult P sum_j G[m][j] u_j; the
     // singlet sums of all orders in one pass over the chi_j
     hipLaunchKernelGGL(k_seed_sums, dim3(nqb, batch, ceil_div(K + 1, kSeedRows)), dim3(256), 0, C.s, chi, cstride,
@@ -371,19 +396,20 @@ extern "C" int rpm_adjoint_sweep(const rpm_operator* op, int batch, void* d_psi,
     c128* lo = rb;   // nu_{n+1}
     RPM_HIP_CHECK(hipMemcpyAsync(hi, z, static_cast<size_t>(C.n) * batch * sizeof(c128), hipMemcpyDeviceToDevice,
                                  C.s));                                                  // nu_K
-    if ((rc = density(C, hi, u[K - 1], (K - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+    if ((rc = density(C, hi, u[K - 1], (K - 1 >= 1 ? 2.0 : 1.0) / C.hs, true))) return rc;
     if ((rc = seed(K - 1))) return rc;
     if ((rc = affine(C, hi, lo, pairs, s2, a2, z, cmake(1.0, 0.0), nullptr, 0.0))) return rc;   // nu_{K-1}
-    if ((rc = density(C, lo, u[K - 2], (K - 2 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+    if ((rc = density(C, lo, u[K - 2], (K - 2 >= 1 ? 2.0 : 1.0) / C.hs, false))) return rc;
     for (int n = K - 2; n >= 1; --n) {
       if ((rc = seed(n))) return rc;
       if ((rc = affine(C, lo, hi, pairs, s2, a2, z, cmake(1.0, 0.0), hi, -1.0))) return rc;     // nu_n
-      if ((rc = density(C, hi, u[n - 1], (n - 1 >= 1 ? 2.0 : 1.0) / C.hs, acc))) return rc;
+      if ((rc = density(C, hi, u[n - 1], (n - 1 >= 1 ? 2.0 : 1.0) / C.hs, false))) return rc;
       std::swap(hi, lo);
     }
     // lo = nu_1, hi = nu_2: costate carried back to the start of the step
     if ((rc = seed(0))) return rc;
     if ((rc = affine(C, lo, g_b, pairs, 1.0 / C.hs, -C.shift / C.hs, z, cmake(1.0, 0.0), hi, -1.0))) return rc;
+    if ((rc = density_finish(C, acc))) return rc;
     std::swap(g_a, g_b);
     std::swap(psi_a, psi_b);
   }
