@@ -698,6 +698,9 @@ def ceiling_probe(h, B: int = 256, reps: int = 20) -> dict:
     return out
 
 
+KERNEL_NAMES = {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 9: "k_local_apply"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -826,7 +829,7 @@ def main() -> int:
                                 "EH D/J=-3, Q on), dim 221184, complex128") if args.operator == "production" else
                                ("batched H·ψ, 14-nucleus random-tensor system (seed 1), dim 221184, complex128"),
                    "batch_vectors": B, "dimension": n,
-                   "kernel": {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 9: "k_local_apply"}[h._device_op().kernel]},
+                   "kernel": KERNEL_NAMES.get(h._device_op().kernel, f"kernel {h._device_op().kernel}")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(B) if args.operator == "production" else None},

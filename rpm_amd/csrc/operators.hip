@@ -212,6 +212,8 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   std::vector<int> ofsO(ngroups * KO > 0 ? ngroups * KO : 1, 0);
   std::vector<c128> coefO(ngroups * KO > 0 ? ngroups * KO * 4 : 4);
   std::vector<unsigned> maskO(ngroups > 0 ? ngroups : 1, 0u);
+  std::vector<unsigned> nzO(ngroups > 0 ? ngroups : 1, 0u), hselO(ngroups > 0 ? ngroups : 1, 0u);
+  unsigned halfO = 0, nzCt = 0;
   if (KO > 16 || KR > 16 || NC > 16) tile_ok = false;
   auto nz = [](c128 c) { return c.re != 0.0 || c.im != 0.0; };
   auto flags4 = [&](const c128* C) {
@@ -284,12 +286,29 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
           for (int e = 0; e < 2; ++e)
             for (int e2 = 0; e2 < 2; ++e2) C[e * 2 + e2] = B[(e * d + m) * (2 * d) + (e2 * d + m2)];
           maskO[g] |= static_cast<unsigned>(flags4(C)) << (2 * k);
+          unsigned bits = 0;
+          for (int q = 0; q < 4; ++q) bits |= nz(C[q]) ? (1u << q) : 0u;
+          nzO[g] |= bits << (4 * k);
+          if ((bits & 10u) != 0 && (bits & 5u) == 0) hselO[g] |= 1u << k;
+          if ((bits & 5u) != 0 && (bits & 10u) != 0) halfO |= 1u << (16 + k);   // both halves somewhere
           ++k;
         }
       }
     }
   }
 
+  {
+    const unsigned both = halfO >> 16;
+    halfO = 0;
+    for (int k = 0; k < KO && k < 16; ++k)
+      if (!((both >> k) & 1u)) halfO |= 1u << k;
+    for (int j = 0; j < NC && tile_ok; ++j)
+      for (int m = 0; m < 2; ++m) {
+        unsigned bits = 0;
+        for (int q = 0; q < 4; ++q) bits |= nz(coefC[(j * 2 + m) * 4 + q]) ? (1u << q) : 0u;
+        nzCt |= bits << (4 * (j * 2 + m));
+      }
+  }
   PairOp* op = new PairOp();
   PairTables& T = op->t;
   T.NL = static_cast<int>(NL);
@@ -307,6 +326,7 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   const size_t o_cC = reserve(coefC.size() * 16);
   const size_t o_oO = reserve(ofsO.size() * 4), o_cO = reserve(coefO.size() * 16), o_fO = reserve(maskO.size() * 4);
   const size_t o_pR = reserve(packR.size() * 4), o_cT = reserve(coefT.size() * 16);
+  const size_t o_nzO = reserve(nzO.size() * 4), o_hsO = reserve(hselO.size() * 4);
   std::vector<char> host(bytes);
   std::memcpy(host.data() + o_cC, coefC.data(), coefC.size() * 16);
   std::memcpy(host.data() + o_oO, ofsO.data(), ofsO.size() * 4);
@@ -314,6 +334,8 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   std::memcpy(host.data() + o_fO, maskO.data(), maskO.size() * 4);
   std::memcpy(host.data() + o_pR, packR.data(), packR.size() * 4);
   std::memcpy(host.data() + o_cT, coefT.data(), coefT.size() * 16);
+  std::memcpy(host.data() + o_nzO, nzO.data(), nzO.size() * 4);
+  std::memcpy(host.data() + o_hsO, hselO.data(), hselO.size() * 4);
   std::memcpy(host.data() + o_dA, dA.data(), dA.size() * 16);
   std::memcpy(host.data() + o_dB, dB.data(), dB.size() * 16);
   std::memcpy(host.data() + o_cA, fA.data(), fA.size() * 16);
@@ -371,6 +393,10 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     U.maskO = reinterpret_cast<const unsigned*>(base + o_fO);
     U.packR = reinterpret_cast<const int*>(base + o_pR);
     U.coefT = reinterpret_cast<const c128*>(base + o_cT);
+    U.nzO = reinterpret_cast<const unsigned*>(base + o_nzO);
+    U.hselO = reinterpret_cast<const unsigned*>(base + o_hsO);
+    U.halfO = halfO;
+    U.nzC = nzCt;
   }
   op->dev = dev;
   *out = op;

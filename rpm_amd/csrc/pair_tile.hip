@@ -10,10 +10,13 @@
 //     (k_pair_tile3) or held in registers (k_pair_tile);
 //   * the other left axes (open, KO neighbour offsets): read from global
 //     memory (L2), the same offsets and coefficients for the whole tile.
-// Every 2x2 electron block of nuclear couplings carries a 2-bit mask so that
-// structurally zero halves (same-electron or electron-flip) are skipped with
-// scalar branches.  Workgroups are mapped XCD-aware: all tiles of a vector run
-// on one XCD, so the re-read neighbours stay in that XCD's L2.
+// Structurally zero couplings are skipped with scalar branches: k_pair_tile
+// masks whole halves (same-electron or electron-flip) of each 2x2 electron
+// block; k_pair_tile3 skips single zero entries of the left blocks (a planar
+// radical's hyperfine tensors, A_xz = A_yz = 0, leave one or two of the four)
+// and loads only the input electron half an open neighbour's nonzero entries
+// read.  Workgroups are mapped XCD-aware: all tiles of a vector run on one XCD,
+// so the re-read neighbours stay in that XCD's L2.
 #include <cstring>
 #include <vector>
 
@@ -21,6 +24,67 @@
 #include "pair_tile.hpp"
 
 namespace rpm {
+
+// Operator tables are never written while a kernel runs: reading them through the constant address
+// space lets the compiler use scalar loads after the kernel's own stores.
+template <typename T>
+using kconst = const __attribute__((address_space(4))) T;
+template <typename T>
+__device__ __forceinline__ kconst<T>* kc(const T* p) {
+  return (kconst<T>*)(p);
+}
+
+__device__ __forceinline__ c128 kload(kconst<c128>* p) { return c128{p->re, p->im}; }
+
+// out[eA,eB] += sum_e' C[eA][e'] q[e',eB], entry eA*2+e' taken when bit (eA*2+e') of nz is set
+// (nz wave-uniform).  h0 = (q[0], q[1]), the input half eA' = 0; h1 = (q[2], q[3]), eA' = 1.
+template <typename CP>
+__device__ __forceinline__ void mac_left_nz(CP C, unsigned nz, const c128* h0, const c128* h1, c128 acc[4]) {
+  if (nz & 1u) {
+    const c128 a = kload(C);
+    acc[0] = cfma(a, h0[0], acc[0]);
+    acc[1] = cfma(a, h0[1], acc[1]);
+  }
+  if (nz & 2u) {
+    const c128 a = kload(C + 1);
+    acc[0] = cfma(a, h1[0], acc[0]);
+    acc[1] = cfma(a, h1[1], acc[1]);
+  }
+  if (nz & 4u) {
+    const c128 a = kload(C + 2);
+    acc[2] = cfma(a, h0[0], acc[2]);
+    acc[3] = cfma(a, h0[1], acc[3]);
+  }
+  if (nz & 8u) {
+    const c128 a = kload(C + 3);
+    acc[2] = cfma(a, h1[0], acc[2]);
+    acc[3] = cfma(a, h1[1], acc[3]);
+  }
+}
+
+// XCD-aware mapping of the linear workgroup id onto (vector, group).
+__device__ inline void tile_map(int G, int batch, int& b, int& grp) {
+  const int g = blockIdx.x;
+  if (batch % kXcds == 0) {
+    const int x = g % kXcds, k = g / kXcds;
+    b = (k / G) * kXcds + x;
+    grp = k % G;
+  } else {
+    b = g / G;
+    grp = g % G;
+  }
+}
+
+// Per-vector control: false when vector b is idle in this launch; s = input scale.
+__device__ inline bool tile_begin(const ApplyCtl& A, int b, double& s) {
+  s = 1.0;
+  if (A.ctl) {
+    const BasisCtl& c = A.ctl[b];
+    if (!basis_step_live(c, A.j)) return false;
+    s = c.scale;
+  }
+  return true;
+}
 
 __device__ inline void mac2x2_left(const c128* __restrict__ C, int flags, const c128 q[4], c128 acc[4]) {
   // out[eA,eB] += sum_e' C[eA][e'] q[e',eB]   (q index e'*2+eB)
@@ -52,30 +116,6 @@ __device__ inline void mac2x2_right(const c128 C[4], int flags, const c128 q[4],
     acc[2] = cfma(C[1], q[3], acc[2]);
     acc[3] = cfma(C[2], q[2], acc[3]);
   }
-}
-
-// XCD-aware mapping of the linear workgroup id onto (vector, group).
-__device__ inline void tile_map(int G, int batch, int& b, int& grp) {
-  const int g = blockIdx.x;
-  if (batch % kXcds == 0) {
-    const int x = g % kXcds, k = g / kXcds;
-    b = (k / G) * kXcds + x;
-    grp = k % G;
-  } else {
-    b = g / G;
-    grp = g % G;
-  }
-}
-
-// Per-vector control: false when vector b is idle in this launch; s = input scale.
-__device__ inline bool tile_begin(const ApplyCtl& A, int b, double& s) {
-  s = 1.0;
-  if (A.ctl) {
-    const BasisCtl& c = A.ctl[b];
-    if (!basis_step_live(c, A.j)) return false;
-    s = c.scale;
-  }
-  return true;
 }
 
 // Register-tiled variant: thread r holds the tile column psi[e][l][r] (4*LG
@@ -201,8 +241,10 @@ __global__ void __launch_bounds__(NR) k_pair_tile(TileTables T, const c128* __re
 
 // LDS-tiled variant (default): one (l, r) per thread, 4 outputs per thread,
 // closed left axes and right neighbours from LDS, right-radical coefficients
-// from a (slot, digit) table staged in LDS.
-template <int NR, int NC, int KO, int KR>
+// from a (slot, digit) table staged in LDS.  HALF marks the open slots that
+// read one input electron half in every group (bit k: slot k; the half is the
+// group's hselO bit).
+template <int NR, int NC, int KO, int KR, unsigned HALF>
 __global__ void __launch_bounds__(NR << NC) __attribute__((amdgpu_waves_per_eu(6, 8)))
 k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, long long ld, int batch, ApplyCtl A) {
   constexpr int LG = 1 << NC;
@@ -217,10 +259,10 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   double s;
   if (!tile_begin(A, b, s)) return;
   const int tid = threadIdx.x;
-  // The vector-memory path (address and data units), not HBM, limits this kernel, so every
-  // per-thread load that is not amplitude data is taken off it: l is wave-uniform (NR is a
-  // multiple of the wave size), so the left tables are scalar loads; the right tables are staged
-  // once per block in LDS; the right neighbour descriptors come in 16-byte pieces.
+  // Vector-memory instructions and FP64 VALU work, not HBM bandwidth, bound this kernel, so every
+  // per-thread load that is not amplitude data is taken off the vector-memory path: l is
+  // wave-uniform (NR is a multiple of the wave size), so the left tables are scalar loads; the right
+  // tables are staged once per block in LDS; the right neighbour descriptors come in 16-byte pieces.
   const int l = __builtin_amdgcn_readfirstlane(tid / NR);
   const int r = tid - l * NR;
   const long long S = T.NLR;
@@ -246,15 +288,25 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   for (int e = 0; e < 4; ++e) acc[e] = cmake(0.0, 0.0);
   {
     const int* __restrict__ ofs = T.ofsO + grp * KO;
-    const c128* __restrict__ cof = T.coefO + static_cast<long long>(grp) * KO * 4;
-    const unsigned fm = T.maskO[grp];
+    kconst<c128>* cof = kc(T.coefO + static_cast<long long>(grp) * KO * 4);
+    const unsigned nzo = T.nzO[grp];
+    const unsigned hs = T.hselO[grp];
 #pragma unroll
     for (int k = 0; k < KO; ++k) {
       const long long o = own + static_cast<long long>(ofs[k]) * NR;
+      const unsigned nzk = (nzo >> (4 * k)) & 0xfu;
       c128 q[4];
+      if ((HALF >> k) & 1u) {
+        // only the input half this group's nonzero entries read
+        const c128* p = x + static_cast<long long>(2 * ((hs >> k) & 1u)) * S + o;
+        q[0] = p[0];
+        q[1] = p[S];
+        mac_left_nz(cof + k * 4, nzk, &q[0], &q[0], acc);
+      } else {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) q[e] = x[e * S + o];
-      mac2x2_left(cof + k * 4, (fm >> (2 * k)) & 3, q, acc);
+        for (int e = 0; e < 4; ++e) q[e] = x[e * S + o];
+        mac_left_nz(cof + k * 4, nzk, &q[0], &q[2], acc);
+      }
     }
   }
 
@@ -282,7 +334,7 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     c128 q[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
-    mac2x2_left(T.coefC + (j * 2 + m) * 4, (T.maskC >> (2 * j)) & 3, q, acc);
+    mac_left_nz(kc(T.coefC) + (j * 2 + m) * 4, (T.nzC >> (4 * (j * 2 + m))) & 0xfu, &q[0], &q[2], acc);
   }
   int pkr[KR];
 #pragma unroll
@@ -344,12 +396,19 @@ int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, 
   return RPM_OK;
 }
 
+
+
 int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
                       hipStream_t stream) {
   if (!pair_tile_supported(T.NR, T.NC, T.KO, T.KR))
     return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
-  hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in, out, ld,
-                     batch, A);
+  constexpr unsigned kHalf = 0x60u;   // open slots 5 and 6 (the two open spin-1/2 methyl protons) read one half
+  if ((T.halfO & kHalf) == kHalf)
+    hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8, kHalf>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in,
+                       out, ld, batch, A);
+  else
+    hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8, 0u>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in,
+                       out, ld, batch, A);
   RPM_LAUNCH_CHECK();
   return RPM_OK;
 }

@@ -27,6 +27,12 @@ struct TileTables {
   const unsigned* maskO;           // [ngroups]  2 bits per open neighbour k
   const int* packR;                // [NR][KR]  (r2 - r) & 0xffff | m << 16 : neighbour offset and digit
   const c128* coefT;               // [KR][3][4] right coefficients by (slot, digit m)
+  // per-entry form of the left couplings: bit eA*2+e' of a 2x2 block's nibble is set when that entry
+  // is nonzero; a half slot reads only the input electron half (eA') that its nonzero entries use
+  const unsigned* nzO;             // [ngroups] one nibble per open neighbour k
+  const unsigned* hselO;           // [ngroups] bit k: the input half a half slot k reads
+  unsigned halfO;                  // bit k: open slot k is a half slot in every group
+  unsigned nzC;                    // one nibble per closed (axis j, digit m) at index j * 2 + m
 };
 
 int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,

@@ -88,7 +88,7 @@ class _DeviceOperator:
         self.handle = 0
 
     def set_kernel(self, kernel: int) -> None:
-        """0 automatic, 1 per-amplitude gather kernel, 2 tiled pair kernel."""
+        """0 automatic, 1 per-amplitude gather kernel, 2 register-tiled, 3 LDS-tiled pair kernel."""
         nat.check(nat.load().rpm_operator_set_kernel(ctypes.c_void_p(self.handle), int(kernel)))
 
     @property
