@@ -86,6 +86,20 @@ __device__ inline bool tile_begin(const ApplyCtl& A, int b, double& s) {
   return true;
 }
 
+// Raw buffer access to one vector's amplitudes: a wave-uniform scalar offset (the electron state's
+// block) plus a 32-bit per-lane byte offset, so each load's address costs one 32-bit add.
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBufferWord3 = 0x00020000;   // raw 32-bit data, gfx9 family buffer resource
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vector_rsrc(const c128* base, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<c128*>(base), 0, static_cast<int>(bytes), kBufferWord3);
+}
+
+__device__ __forceinline__ c128 bld16(__amdgpu_buffer_rsrc_t rs, unsigned voff, unsigned soff) {
+  const i32x4 v = __builtin_amdgcn_raw_buffer_load_b128(rs, static_cast<int>(voff), static_cast<int>(soff), 0);
+  return __builtin_bit_cast(c128, v);
+}
+
 __device__ inline void mac2x2_left(const c128* __restrict__ C, int flags, const c128 q[4], c128 acc[4]) {
   // out[eA,eB] += sum_e' C[eA][e'] q[e',eB]   (q index e'*2+eB)
   if (flags & kSameE) {
@@ -269,6 +283,11 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   const c128* __restrict__ x = in + b * ld;
   const int l0 = grp * LG;
   const long long own = static_cast<long long>(l0 + l) * NR + r;
+  // Amplitude loads go through a buffer resource over the vector (4 S amplitudes, under 2^31 bytes):
+  // electron state e at scalar offset e * Sb, the lane's amplitude at vector offset ob.
+  const long long Sb = S * static_cast<long long>(sizeof(c128));
+  const __amdgpu_buffer_rsrc_t xr = vector_rsrc(x, 4 * Sb);
+  const unsigned ob = static_cast<unsigned>(own * static_cast<long long>(sizeof(c128)));
 
   for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
   for (int i = tid; i < 4 * NR; i += NT) {
@@ -278,7 +297,7 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   c128 psi[4];
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    psi[e] = x[e * S + own];
+    psi[e] = bld16(xr, ob, static_cast<unsigned>(e * Sb));
     tile[(e * LG + l) * NR + r] = psi[e];
   }
 
@@ -293,18 +312,18 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     const unsigned hs = T.hselO[grp];
 #pragma unroll
     for (int k = 0; k < KO; ++k) {
-      const long long o = own + static_cast<long long>(ofs[k]) * NR;
+      const unsigned o = ob + static_cast<unsigned>(ofs[k] * NR * static_cast<int>(sizeof(c128)));
       const unsigned nzk = (nzo >> (4 * k)) & 0xfu;
       c128 q[4];
       if ((HALF >> k) & 1u) {
         // only the input half this group's nonzero entries read
-        const c128* p = x + static_cast<long long>(2 * ((hs >> k) & 1u)) * S + o;
-        q[0] = p[0];
-        q[1] = p[S];
+        const unsigned h = 2 * ((hs >> k) & 1u);
+        q[0] = bld16(xr, o, static_cast<unsigned>(h * Sb));
+        q[1] = bld16(xr, o, static_cast<unsigned>((h + 1) * Sb));
         mac_left_nz(cof + k * 4, nzk, &q[0], &q[0], acc);
       } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) q[e] = x[e * S + o];
+        for (int e = 0; e < 4; ++e) q[e] = bld16(xr, o, static_cast<unsigned>(e * Sb));
         mac_left_nz(cof + k * 4, nzk, &q[0], &q[2], acc);
       }
     }
@@ -358,20 +377,22 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
     mac2x2_right(C, (T.maskR >> (2 * k)) & 3, q, acc);
   }
 
-  c128* __restrict__ y = out + b * ld;
+  char* __restrict__ yb = reinterpret_cast<char*>(out + b * ld);
   const double bprev = A.prev ? A.ctl[b].beta_prev : 0.0;
+  const bool plain = !A.ctl && !A.affine;   // s == 1: the output is H in itself
   c128 dot{0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < 4; ++e) {
-    c128 v = apply_epilogue(A, acc[e], psi[e], s, b, ld, e * S + own);
+    c128 v = plain ? acc[e] : apply_epilogue(A, acc[e], psi[e], s, b, ld, e * S + own);
     if (A.dots) dot = cfmaconj(cscale(psi[e], s), v, dot);
     if (A.prev) {
       const c128 p = A.prev[b * A.prev_ld + e * S + own];
       v.re = fma(-bprev, p.re, v.re);
       v.im = fma(-bprev, p.im, v.im);
     }
-    __builtin_nontemporal_store(v.re, &y[e * S + own].re);
-    __builtin_nontemporal_store(v.im, &y[e * S + own].im);
+    c128* yo = reinterpret_cast<c128*>(yb + e * Sb + ob);
+    __builtin_nontemporal_store(v.re, &yo->re);
+    __builtin_nontemporal_store(v.im, &yo->im);
   }
   if (A.copy) {
     c128* z = A.copy + b * A.copy_ld;
