@@ -350,10 +350,19 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   for (int j = 0; j < NC; ++j) {
     const int l2 = l ^ (1 << j);
     const int m = (l >> j) & 1;
+    const unsigned nzj = (T.nzC >> (4 * (j * 2 + m))) & 0xfu;
     c128 q[4];
+    if ((nzj & 5u) == 0 || (nzj & 10u) == 0) {
+      // the nonzero entries read one input electron half (l, hence m, is wave-uniform)
+      const int h = (nzj & 10u) ? 2 : 0;
+      q[0] = tile[(h * LG + l2) * NR + r];
+      q[1] = tile[((h + 1) * LG + l2) * NR + r];
+      mac_left_nz(kc(T.coefC) + (j * 2 + m) * 4, nzj, &q[0], &q[0], acc);
+    } else {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
-    mac_left_nz(kc(T.coefC) + (j * 2 + m) * 4, (T.nzC >> (4 * (j * 2 + m))) & 0xfu, &q[0], &q[2], acc);
+      for (int e = 0; e < 4; ++e) q[e] = tile[(e * LG + l2) * NR + r];
+      mac_left_nz(kc(T.coefC) + (j * 2 + m) * 4, nzj, &q[0], &q[2], acc);
+    }
   }
   int pkr[KR];
 #pragma unroll
