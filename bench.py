@@ -366,6 +366,7 @@ def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
     states = torch.from_numpy(np.stack(states)).cuda()
     pairs = batch.pair_matrices(h, hs)
     bp = batch.BatchedLanczosPropagator.like(core.make_propagator(h, cfg, probe_horizon=float(times[-1])))
+    bp.use_graphs = bool(getattr(args, "graphs", False))
     bp.propagate(states, times[:3], pairs=pairs)      # warm-up at the full batch (allocations, first launches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -396,7 +397,8 @@ def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
             "trajectory_us_per_s_plain": plain_rate, "speedup_vs_plain": batched_rate / plain_rate,
             "yields_per_s_batched_equiv": batched_rate / cfg.maximum_time_us,
             "yields_per_s_plain_equiv": plain_rate / cfg.maximum_time_us,
-            "full_horizon_us": cfg.maximum_time_us, "matvecs": matvecs, "rounds": bp.rounds}
+            "full_horizon_us": cfg.maximum_time_us, "matvecs": matvecs, "rounds": bp.rounds,
+            "graphs": bp.use_graphs}
 
 
 def directional_probe(h, args, n_dirs=64, fields_uT=(25.0, 50.0), n_states=2, horizon=0.25) -> dict:
@@ -715,6 +717,8 @@ def main() -> int:
                     help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
+    ap.add_argument("--graphs", action="store_true",
+                    help="batch mode: replay each window of basis steps as a captured HIP graph")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
                              "trajectory", "archive_batch", "gradcheck", "pulse", "ceiling"),
