@@ -846,7 +846,7 @@ def main() -> int:
         # the same kernel at other batch sizes: B <= 32 fits the 256 MB Infinity Cache, B >= 128 is HBM-bound
         del y
         sweep = {}
-        for bs in (16, 64, 1024):
+        for bs in (1, 16, 64, 1024):
             if bs <= B:
                 xs = x[:bs]
             else:   # synthetic data drawn on the device (the values do not affect the timing)
@@ -870,6 +870,28 @@ def main() -> int:
                               "frac": gbs / HBM_PEAK_GBS}
             del xs, ys
         result["hpsi_sweep"] = sweep
+        if args.operator == "production" and B <= x.shape[0]:
+            # the configuration-1 operator family (random, fully anisotropic tensors on all 14 nuclei, no
+            # structurally zero couplings) on the same resident vectors
+            hr = workloads.full_random_system(seed=1).hamiltonian()
+            ys = torch.empty_like(x)
+            for _ in range(3):
+                hr.apply_device(x, ys, batch=B, ld=n)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            reps = 20
+            e0.record()
+            for _ in range(reps):
+                hr.apply_device(x, ys, batch=B, ld=n)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            gbs = BYTES_PER_AMPLITUDE * n * B / (ms / 1e3) / 1e9
+            result["hpsi_random_operator"] = {"batch_vectors": B, "ms_per_launch": ms, "hpsi_per_s": B / (ms / 1e3),
+                                              "achieved_GBps": gbs, "frac": gbs / HBM_PEAK_GBS,
+                                              "kernel": KERNEL_NAMES.get(hr._device_op().kernel, "?"),
+                                              "operator": "14 nuclei, random hyperfine and quadrupole tensors, seed 1"}
+            del ys, hr
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]
