@@ -432,6 +432,9 @@ int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch,
                       hipStream_t stream) {
   if (!pair_tile_supported(T.NR, T.NC, T.KO, T.KR))
     return fail(RPM_E_UNSUPPORTED, "no tiled pair kernel for NR=%d NC=%d KO=%d KR=%d", T.NR, T.NC, T.KO, T.KR);
+  // the amplitude reads address a vector through a buffer resource with a 32-bit size
+  if (4 * T.NLR * static_cast<long long>(sizeof(c128)) > 0x7fffffffLL)
+    return fail(RPM_E_UNSUPPORTED, "vector of %lld amplitudes too large for the tiled kernel", 4 * T.NLR);
   constexpr unsigned kHalf = 0x60u;   // open slots 5 and 6 (the two open spin-1/2 methyl protons) read one half
   if ((T.halfO & kHalf) == kHalf)
     hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8, kHalf>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in,
