@@ -415,6 +415,31 @@ def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch, k
         assert _rel(yg[b], want) <= HPSI_TOL, b
 
 
+def test_tiled_kernel_mixed_zero_pattern_matches_oracle():
+    """Production tensors with one open methyl proton (H8a) tilted out of the radical plane
+    (A_xz, A_yz != 0): its neighbour slot reads both electron halves, so the kernel takes its generic
+    form, while every other left coupling keeps its zero entries (skipped per entry)."""
+    import dataclasses
+    print("case: production with H8a tilted, tiled kernel vs oracle", flush=True)
+    base = workloads.production_system()
+    nuclei = list(base.nuclei)
+    i = [n.name for n in nuclei].index("H8a")
+    a = np.array(nuclei[i].tensor_mT, dtype=float)
+    a[0, 2] = a[2, 0] = 0.05
+    a[1, 2] = a[2, 1] = -0.03
+    nuclei[i] = dataclasses.replace(nuclei[i], tensor_mT=a)
+    tilted = dataclasses.replace(base, nuclei=tuple(nuclei))
+    h = tilted.hamiltonian()
+    o = tilted.hamiltonian(builder=ocore.build_hamiltonian)
+    assert h._device_op().kernel == 3
+    v = workloads.normal_vectors(h.dimension, 2, seed=12)
+    for b in range(2):
+        assert _rel(h.matvec(v[b]), o.matvec(v[b])) <= HPSI_TOL
+    # the tilt changes H·psi: the test is not passing on the unchanged production operator
+    p = workloads.production_system().hamiltonian(builder=ocore.build_hamiltonian)
+    assert _rel(p.matvec(v[0]), o.matvec(v[0])) > 1e-4
+
+
 def test_tiled_kernel_random_tensor_system():
     sysr = workloads.full_random_system(seed=7)
     h = sysr.hamiltonian(theta_deg=63.0, phi_deg=20.0)
