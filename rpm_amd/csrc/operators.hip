@@ -213,8 +213,10 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   std::vector<c128> coefO(ngroups * KO > 0 ? ngroups * KO * 4 : 4);
   std::vector<unsigned> maskO(ngroups > 0 ? ngroups : 1, 0u);
   std::vector<unsigned> nzO(ngroups > 0 ? ngroups : 1, 0u), hselO(ngroups > 0 ? ngroups : 1, 0u);
+  unsigned bothO = 0;   // bit k: some group's open slot k reads both input electron halves
   unsigned halfO = 0, nzCt = 0;
-  if (KO > 16 || KR > 16 || NC > 16) tile_ok = false;
+  // one nibble per open slot in a 32-bit word (nzO) and per closed (axis, digit) pair (nzC)
+  if (KO > 8 || KR > 16 || NC > 4) tile_ok = false;
   auto nz = [](c128 c) { return c.re != 0.0 || c.im != 0.0; };
   auto flags4 = [&](const c128* C) {
     int f = 0;
@@ -290,19 +292,17 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
           for (int q = 0; q < 4; ++q) bits |= nz(C[q]) ? (1u << q) : 0u;
           nzO[g] |= bits << (4 * k);
           if ((bits & 10u) != 0 && (bits & 5u) == 0) hselO[g] |= 1u << k;
-          if ((bits & 5u) != 0 && (bits & 10u) != 0) halfO |= 1u << (16 + k);   // both halves somewhere
+          if ((bits & 5u) != 0 && (bits & 10u) != 0) bothO |= 1u << k;
           ++k;
         }
       }
     }
   }
 
-  {
-    const unsigned both = halfO >> 16;
-    halfO = 0;
-    for (int k = 0; k < KO && k < 16; ++k)
-      if (!((both >> k) & 1u)) halfO |= 1u << k;
-    for (int j = 0; j < NC && tile_ok; ++j)
+  if (tile_ok) {
+    for (int k = 0; k < KO; ++k)
+      if (!((bothO >> k) & 1u)) halfO |= 1u << k;
+    for (int j = 0; j < NC; ++j)
       for (int m = 0; m < 2; ++m) {
         unsigned bits = 0;
         for (int q = 0; q < 4; ++q) bits |= nz(coefC[(j * 2 + m) * 4 + q]) ? (1u << q) : 0u;
