@@ -1232,8 +1232,10 @@ __global__ void __launch_bounds__(kThreads) k_combine(const c128* __restrict__ V
 constexpr int kGramTile = 64;
 constexpr int kGramChunk = 16;
 
-// Partial S[a][b'] = sum_n conj(A_a[n]) A_b'[n] for a, b' < k <= 64, A_a[n] = (V_a[nn + n] - V_a[2 nn + n]) / sqrt2.
-// One 4x4 register block per thread; grid (nblk, batch).
+// Partial S[a][c] = sum_n conj(A_a[n]) A_c[n] for the 4x4 blocks on and above the diagonal of the k x k
+// matrix, k <= 64, A_a[n] = (V_a[nn + n] - V_a[2 nn + n]) / sqrt2 (k_gram_sym completes the Hermitian
+// matrix). Thread t < T (T + 1) / 2, T = ceil(k / 4), computes the t-th such block (ia, ib), ia <= ib,
+// in row order; every thread stages. Entries below the diagonal blocks are not written. Grid (nblk, batch).
 __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long long vstride, const int* __restrict__ kb,
                                               int k1, long long nn, long long n_per_block, c128* __restrict__ partial) {
   const int bt = blockIdx.y;
@@ -1241,7 +1243,17 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long l
   if (k <= 0) return;
   __shared__ c128 ra[kGramChunk][kGramTile + 1];
   const int tid = threadIdx.x;
-  const int ia = tid / 16, ib = tid % 16;
+  const int T4 = (k + 3) / 4;
+  const bool act = tid < T4 * (T4 + 1) / 2;
+  int ia = 0, ib = 0;
+  if (act) {
+    int q = tid;
+    while (q >= T4 - ia) {
+      q -= T4 - ia;
+      ++ia;
+    }
+    ib = ia + q;
+  }
   const long long N = 4 * nn;
   const c128* Vb = V + bt * vstride;
   const long long n0 = static_cast<long long>(blockIdx.x) * n_per_block;
@@ -1265,7 +1277,7 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long l
       ra[col][row] = va;
     }
     __syncthreads();
-    for (int col = 0; col < kGramChunk; ++col) {
+    for (int col = 0; col < kGramChunk && act; ++col) {
       c128 xa[4], xb[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
@@ -1279,6 +1291,7 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long l
     }
     __syncthreads();
   }
+  if (!act) return;
   c128* out = partial + (static_cast<long long>(bt) * gridDim.x + blockIdx.x) * kGramTile * kGramTile;
 #pragma unroll
   for (int u = 0; u < 4; ++u)
@@ -1286,9 +1299,10 @@ __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long l
     for (int v = 0; v < 4; ++v) out[(ia * 4 + u) * kGramTile + ib * 4 + v] = acc[u][v];
 }
 
-// P_b = Hermitian part of the sum over blocks of the partial tiles (k x k row-major at P + b*pstride),
-// in two steps so that every read is coalesced: k_gram_sum adds the tiles (each entry over the blocks
-// in ascending order) into P, k_gram_sym replaces each pair (a, c), (c, a) by the Hermitian part.
+// P_b = the Hermitian matrix whose upper triangle is the sum over blocks of the partial tiles (k x k
+// row-major at P + b*pstride), in two steps so that every read is coalesced: k_gram_sum adds the tiles
+// (each entry a <= c over the blocks in ascending order) into P, k_gram_sym sets P[c][a] = conj(P[a][c])
+// and takes the real part of the diagonal.
 __global__ void __launch_bounds__(256) k_gram_sum(const c128* __restrict__ partial, int nblk,
                                                   const int* __restrict__ kb, int k1, c128* __restrict__ P,
                                                   long long pstride) {
@@ -1297,6 +1311,7 @@ __global__ void __launch_bounds__(256) k_gram_sum(const c128* __restrict__ parti
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= k * k) return;
   const int a = e / k, c = e % k;
+  if (a > c) return;
   const c128* pb = partial + static_cast<long long>(bt) * nblk * kGramTile * kGramTile + a * kGramTile + c;
   c128 x{0.0, 0.0};
   // unrolled so that the loads run ahead of the (ordered) additions
@@ -1314,9 +1329,12 @@ __global__ void __launch_bounds__(256) k_gram_sym(const int* __restrict__ kb, in
   const int a = e / k, c = e % k;
   if (a > c) return;
   c128* pb = P + bt * pstride;
-  const c128 x = pb[a * k + c], y = pb[c * k + a];
-  pb[a * k + c] = cmake(0.5 * (x.re + y.re), 0.5 * (x.im - y.im));
-  pb[c * k + a] = cmake(0.5 * (y.re + x.re), 0.5 * (y.im - x.im));
+  const c128 x = pb[a * k + c];
+  if (a == c) {
+    pb[a * k + a] = cmake(x.re, 0.0);
+  } else {
+    pb[c * k + a] = cmake(x.re, -x.im);
+  }
 }
 
 }  // namespace rpm
