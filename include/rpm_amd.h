@@ -241,8 +241,8 @@ int rpm_yield_accumulate(const double* d_values, const double* d_point_err, long
                          const long long* d_spans, int batch, const double* d_weights, long long nweights,
                          const double* d_acc, double* d_phi, void* stream);
 
-/* Batched rpm_singlet_gram: P_b (k_b x k_b at d_P + b*m*m) of the first k_b = d_k[b]
- * vectors of basis b (d_basis + b*m*N); k_b <= 0 skips b. */
+/* Batched rpm_singlet_gram: P_b (k_b x k_b at d_P + b*m*m) of the first k_b = min(d_k[b], m)
+ * vectors of basis b (d_basis + b*m*N), m <= 64; k_b <= 0 skips b. */
 size_t rpm_singlet_gram_batch_workspace_bytes(long long nn, int batch);
 int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k, int batch, long long nn, void* d_P,
                            void* d_workspace, size_t workspace_bytes, void* stream);

@@ -1239,7 +1239,7 @@ constexpr int kGramChunk = 16;
 __global__ void __launch_bounds__(256) k_gram(const c128* __restrict__ V, long long vstride, const int* __restrict__ kb,
                                               int k1, long long nn, long long n_per_block, c128* __restrict__ partial) {
   const int bt = blockIdx.y;
-  const int k = kb ? kb[bt] : k1;
+  const int k = kb ? min(kb[bt], k1) : k1;   // per-trajectory counts are clamped to the basis
   if (k <= 0) return;
   __shared__ c128 ra[kGramChunk][kGramTile + 1];
   const int tid = threadIdx.x;
@@ -1307,7 +1307,7 @@ __global__ void __launch_bounds__(256) k_gram_sum(const c128* __restrict__ parti
                                                   const int* __restrict__ kb, int k1, c128* __restrict__ P,
                                                   long long pstride) {
   const int bt = blockIdx.y;
-  const int k = kb ? kb[bt] : k1;
+  const int k = kb ? min(kb[bt], k1) : k1;   // per-trajectory counts are clamped to the basis
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= k * k) return;
   const int a = e / k, c = e % k;
@@ -1323,7 +1323,7 @@ __global__ void __launch_bounds__(256) k_gram_sum(const c128* __restrict__ parti
 __global__ void __launch_bounds__(256) k_gram_sym(const int* __restrict__ kb, int k1, c128* __restrict__ P,
                                                   long long pstride) {
   const int bt = blockIdx.y;
-  const int k = kb ? kb[bt] : k1;
+  const int k = kb ? min(kb[bt], k1) : k1;   // per-trajectory counts are clamped to the basis
   const int e = blockIdx.x * 256 + threadIdx.x;
   if (e >= k * k) return;
   const int a = e / k, c = e % k;
@@ -1837,7 +1837,7 @@ extern "C" int rpm_singlet_gram_batch(const void* d_basis, int m, const int* d_k
                                       void* d_ws, size_t ws_bytes, void* stream) {
   if (!d_basis || !d_k || !d_P || !d_ws || batch < 1 || batch > 65535 || nn < 1 || m < 1 || m > kGramTile)
     return fail(RPM_E_INVALID, "rpm_singlet_gram_batch: bad arguments");
-  return gram_batch(static_cast<const c128*>(d_basis), static_cast<long long>(m) * 4 * nn, d_k, 0, batch, nn,
+  return gram_batch(static_cast<const c128*>(d_basis), static_cast<long long>(m) * 4 * nn, d_k, m, batch, nn,
                     static_cast<c128*>(d_P), static_cast<long long>(m) * m, d_ws, ws_bytes,
                     static_cast<hipStream_t>(stream));
 }
