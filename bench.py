@@ -892,6 +892,26 @@ def main() -> int:
                                               "kernel": KERNEL_NAMES.get(hr._device_op().kernel, "?"),
                                               "operator": "14 nuclei, random hyperfine and quadrupole tensors, seed 1"}
             del ys, hr
+        # the drop-in API with host (NumPy) buffers: host layout copies, PCIe both ways and the kernel
+        # (reported beside the resident-vector value, never as it)
+        host = {}
+        rng = np.random.default_rng(3)
+        for name, cols in (("matvec", 0), ("matmat_16", 16)):
+            shape = (n,) if cols == 0 else (n, cols)
+            hv = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+            call = h.matvec if cols == 0 else h.matmat
+            call(hv)
+            torch.cuda.synchronize()
+            reps = 5
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call(hv)
+            dt = (time.perf_counter() - t0) / reps
+            k = max(cols, 1)
+            host[name] = {"vectors": k, "ms_per_call": dt * 1e3, "hpsi_per_s": k / dt,
+                          "GBps_algorithmic": BYTES_PER_AMPLITUDE * n * k / dt / 1e9}
+            del hv
+        result["hpsi_host_buffers"] = host
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline()
         result["gpu_over_cpu"] = value / result["cpu_baseline"]["value"]

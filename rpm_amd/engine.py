@@ -173,7 +173,8 @@ class _GpuHamiltonianBase:
         if nat.is_torch(values):
             x = nat.to_device_c128(values.T.contiguous())
         else:
-            x = nat.to_device_c128(np.ascontiguousarray(values.T))
+            # upload the host layout as it is and transpose on the device (a host transpose costs more)
+            x = nat.to_device_c128(values).T.contiguous()
         ncol = values.shape[1]
         out = nat.empty_c128(ncol, self.dimension)
         for start in range(0, ncol, 65535):
@@ -181,7 +182,7 @@ class _GpuHamiltonianBase:
             self.apply_device(x[start:stop], out[start:stop], batch=stop - start, ld=self.dimension)
         self.matvec_count += ncol
         res = out.T
-        return res if nat.is_torch(matrix) else res.cpu().numpy()
+        return res if nat.is_torch(matrix) else res.contiguous().cpu().numpy()
 
     def aslinearoperator(self):
         from scipy.sparse.linalg import LinearOperator
