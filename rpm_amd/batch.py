@@ -470,6 +470,9 @@ class BatchedLanczosPropagator:
                         launch(nat.stream_ptr(), j0=int(min(inf_h[b, 0] for b in short)), max_stop=int(ts[B:].max()))
                         inf_new = info.cpu().numpy()
                         for b in short:
+                            if inf_new[b, 3] and inf_new[b, 0] <= inf_h[b, 0]:
+                                raise RuntimeError(f"Lanczos basis of trajectory {b}: continuing from "
+                                                   f"{int(inf_h[b, 0])} vectors made no progress")
                             inf_h[b] = inf_new[b]
                     nrm_h = norm.cpu().numpy()
                     for b in group:

@@ -764,10 +764,13 @@ class AdaptiveLanczosPropagator:
         size = int(info[0])
         while info[3]:
             # a paired step stopped the basis early at a near-breakdown: continue it to the stop
+            before = size
             ws.targets.copy_(torch.from_numpy(np.array([0, ts[1]], dtype=np.int32)))
             launch(nat.stream_ptr(), j0=size)
             info = ws.info.cpu().numpy()
             size = int(info[0])
+            if info[3] and size <= before:
+                raise RuntimeError(f"Lanczos basis: continuing from {before} vectors made no progress")
         self.device_matvecs += size - j0
         self._built = size
         return size
