@@ -366,7 +366,7 @@ def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
     states = torch.from_numpy(np.stack(states)).cuda()
     pairs = batch.pair_matrices(h, hs)
     bp = batch.BatchedLanczosPropagator.like(core.make_propagator(h, cfg, probe_horizon=float(times[-1])))
-    bp.use_graphs = bool(getattr(args, "graphs", False))
+    bp.use_graphs = not bool(getattr(args, "eager", False))
     bp.propagate(states, times[:3], pairs=pairs)      # warm-up at the full batch (allocations, first launches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -717,8 +717,9 @@ def main() -> int:
                     help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
-    ap.add_argument("--graphs", action="store_true",
-                    help="batch mode: replay each window of basis steps as a captured HIP graph")
+    ap.add_argument("--eager", action="store_true",
+                    help="batch mode: launch each window of basis steps eagerly instead of replaying it as a "
+                         "captured HIP graph (the default)")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
                              "trajectory", "archive_batch", "gradcheck", "pulse", "ceiling"),

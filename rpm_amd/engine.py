@@ -462,22 +462,33 @@ def device_norm(x, out=None):
 class BasisGraphs:
     """HIP graphs of the basis builder's launch sequences, one per (first step, last step) window.
 
-    The builder launches kernels only (no allocation, host synchronisation or memset), so each window
-    is captured once (on a side stream, torch.cuda.graph) and replayed on the current stream; the
-    per-call targets and stops are read from device memory at replay. A cache belongs to one set of
-    buffers: build a new one when the buffers change.
+    The builder launches kernels only (no allocation, host synchronisation or memset), so a window
+    is captured (on a side stream, torch.cuda.graph) the ``min_uses``-th time it is run and replayed on
+    the current stream from then on; the per-call targets and stops are read from device memory at
+    replay. Earlier runs of a window launch eagerly, so windows that a short propagation meets once
+    or twice pay no capture. A cache belongs to one set of buffers: build a new one when the buffers
+    change.
     """
 
-    def __init__(self, owner=None):
+    def __init__(self, owner=None, min_uses: int = 3):
         self.owner = owner          # the buffers the graphs were captured with
+        self.min_uses = max(1, int(min_uses))
         self.graphs = {}
+        self.uses = {}
         self.captures = 0
         self.replays = 0
+        self.eager_runs = 0
 
     def run(self, key, launch):
         """``launch(stream_handle)`` issues the window's kernels on the given stream."""
         torch = nat.require_gpu()
+        uses = self.uses.get(key, 0) + 1
+        self.uses[key] = uses
         g = self.graphs.get(key)
+        if g is None and uses < self.min_uses:
+            launch(torch.cuda.current_stream().cuda_stream)
+            self.eager_runs += 1
+            return
         if g is None:
             g = torch.cuda.CUDAGraph()
             cur = torch.cuda.current_stream()

@@ -32,10 +32,11 @@ def test_batched_graph_replay_equals_eager():
     w = quadrature.yield_weight_table(times, 5.0)
     kw = dict(state_error_budget=1e-9)
     eager = batch.BatchedLanczosPropagator(h, **kw)
+    eager.use_graphs = False
     ref = eager.propagate(states, times, pairs=pairs, yield_weights=w)
     ref_final = eager.final_states.clone()
     bp = batch.BatchedLanczosPropagator(h, **kw)
-    bp.use_graphs = True
+    assert bp.use_graphs          # the default
     got = bp.propagate(states, times, pairs=pairs, yield_weights=w)
     print("captures", bp.graphs.captures, "replays", bp.graphs.replays, flush=True)
     assert bp.graphs.captures >= 1 and bp.graphs.replays > bp.graphs.captures
