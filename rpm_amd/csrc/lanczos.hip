@@ -549,14 +549,27 @@ __device__ inline int pair_mode(const BasisCtl& C, int j) {
 }
 
 // Input of M2: z / s for a pair (M2 scales its input by s, so it forms H z), x_{j+1} for S1.
+// With dots (a single trajectory), every block forms alpha~_j from M1's per-unit partials in the
+// order of k_alpha, and block 0 stores it, in place of a k_alpha launch.
 __global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__ V, long long vstride, long long n,
                                                         int j, const c128* __restrict__ X0, const c128* __restrict__ X1,
                                                         const c128* __restrict__ ZA, c128* __restrict__ Z,
-                                                        const double* __restrict__ atilde, const BasisCtl* ctl) {
+                                                        double* __restrict__ atilde, const c128* __restrict__ dots,
+                                                        int units, const BasisCtl* ctl) {
   const int b = blockIdx.y;
   const BasisCtl& C = ctl[b];
   const int mode = pair_mode(C, j);
   if (mode != kModePair && mode != kModeS1) return;
+  double at = 0.0;
+  if (mode == kModePair) {
+    if (dots) {
+      __shared__ c128 sm[kThreads / kWave];
+      at = alpha_sum(dots, units, b, sm);
+      if (blockIdx.x == 0 && threadIdx.x == 0) atilde[b] = at;
+    } else {
+      at = atilde[b];
+    }
+  }
   const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
   if (i >= n) return;
   const long long o = b * n + i;
@@ -564,7 +577,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__
     Z[o] = X1[o];
     return;
   }
-  const double s = C.scale, at = atilde[b], bp = C.beta_prev;
+  const double s = C.scale, bp = C.beta_prev;
   const c128 xh = cscale(X0[o], s);
   const c128 xp = C.npend == 2 ? cscale(ZA[o], C.zscale) : V[b * vstride + static_cast<long long>(j - 1) * n + i];
   c128 z = X1[o];
@@ -773,14 +786,37 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
                                                         const c128* __restrict__ ZA, const c128* __restrict__ Z,
                                                         const c128* __restrict__ Q, const c128* __restrict__ cpendA,
                                                         const c128* __restrict__ cpend, int cstride,
-                                                        const double* __restrict__ atilde,
-                                                        const double* __restrict__ atilde2, c128* __restrict__ partial,
+                                                        double* __restrict__ atilde,
+                                                        double* __restrict__ atilde2, const c128* __restrict__ dots,
+                                                        const c128* __restrict__ dots2, int units,
+                                                        c128* __restrict__ partial,
                                                         long long pstride, const BasisCtl* ctl, int blk0, int nblk) {
   static_assert(kPassElems == kThreads, "one element per thread outside the column stream");
   const int b = blockIdx.y;
   const BasisCtl& C = ctl[b];
   const int mode = pair_mode(C, j);
   if (mode == kModeIdle) return;
+  // With dots and dots2 (a single trajectory; k_alpha does not run), the alpha~ this pass or the
+  // slot's finish needs comes from the matvec partials, summed in the order of k_alpha: a pair's
+  // alpha~_j was stored by k_pair_prep and its alpha~_{j+1} (finish) is stored by block 0 here; a
+  // single step S0 or S1 forms its own in every block, and block 0 stores it.
+  double at_single = 0.0;   // S0 / S1 with dots: this block's alpha~ (never re-read from memory here)
+  if (dots) {
+    __shared__ c128 s_al[kThreads / kWave];
+    const bool lead = blk0 + blockIdx.x == 0;
+    if (mode == kModePair) {
+      if (lead) {
+        const double a2 = alpha_sum(dots2, units, b, s_al);
+        if (threadIdx.x == 0) atilde2[b] = a2;
+      }
+    } else if (mode == kModeS0) {
+      at_single = alpha_sum(dots, units, b, s_al);
+      if (lead && threadIdx.x == 0) atilde[b] = at_single;
+    } else {
+      at_single = alpha_sum(dots2, units, b, s_al);
+      if (lead && threadIdx.x == 0) atilde2[b] = at_single;
+    }
+  }
   __shared__ c128 s_r[kPassElems];
   __shared__ c128 s_q[kPassElems];
   __shared__ c128 s_x[kPassElems];
@@ -821,7 +857,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
     P.xb = X0 + b * n;
     P.r = X1 + b * n;      // P = H x^_j
     P.y = X1 + b * n;      // y_j, the pending x_{j+1}, over P (each element read before it is written)
-    P.at = atilde[b];
+    P.at = dots ? at_single : atilde[b];
     if (np2)
       pair_pass_body<kStream, kFull, false, true>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
     else
@@ -831,7 +867,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
     P.xb = X1 + b * n;
     P.r = Q + b * n;       // M2 on x_{j+1}
     P.y = X0 + b * n;
-    P.at = atilde2[b];
+    P.at = dots ? at_single : atilde2[b];
     pair_pass_body<kStream, kFull, false, false>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
   }
 }
@@ -1570,36 +1606,38 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
     A.dots_ld = w.units;
     int rc = operator_apply_ex(op, w.w[0], w.w[1], batch, n, A, s);   // P = H x^_j into w[1]
     if (rc) return rc;
-    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, w.atilde, j, w.ctl);
-    RPM_LAUNCH_CHECK();
+    // A single trajectory forms both alpha~ sums inside the prep and the pass (two launches fewer
+    // per slot); a batch runs k_alpha. Both sum in the same order: the bases stay bitwise equal.
+    const bool fuse_alpha = batch == 1;
+    const c128* fdots = fuse_alpha ? w.dots : nullptr;
+    const c128* fdots2 = fuse_alpha ? w.dots2 : nullptr;
+    if (!fuse_alpha) {
+      hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, w.atilde, j, w.ctl);
+      RPM_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(k_pair_prep, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                       w.w[0], w.w[1], zp, zc, w.atilde, w.ctl);
+                       w.w[0], w.w[1], zp, zc, w.atilde, fdots, w.units, w.ctl);
     RPM_LAUNCH_CHECK();
     A.j = j + 1;
     A.dots = w.dots2;
     rc = operator_apply_ex(op, zc, w.q, batch, n, A, s);               // Q = H z (S1: H x^_{j+1})
     if (rc) return rc;
-    hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots2, w.units, w.atilde2, j + 1, w.ctl);
-    RPM_LAUNCH_CHECK();
+    if (!fuse_alpha) {
+      hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots2, w.units, w.atilde2, j + 1, w.ctl);
+      RPM_LAUNCH_CHECK();
+    }
     const int nfull = static_cast<int>(n / kPassElems);
+    auto pass = [&](auto kernel, int b0, int nb) {
+      hipLaunchKernelGGL(kernel, dim3(nb, batch), dim3(kThreads), 0, s, V, vstride, n, j, w.w[0], w.w[1], zp, zc,
+                         w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, fdots, fdots2, w.units, w.partial, w.pstride,
+                         w.ctl, b0, w.npass);
+    };
     if (batch > 1) {
-      if (nfull > 0)
-        hipLaunchKernelGGL((k_pair_pass<true, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                           w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial, w.pstride,
-                           w.ctl, 0, w.npass);
-      if (nfull < w.npass)
-        hipLaunchKernelGGL((k_pair_pass<true, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
-                           n, j, w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial,
-                           w.pstride, w.ctl, nfull, w.npass);
+      if (nfull > 0) pass(k_pair_pass<true, true>, 0, nfull);
+      if (nfull < w.npass) pass(k_pair_pass<true, false>, nfull, w.npass - nfull);
     } else {
-      if (nfull > 0)
-        hipLaunchKernelGGL((k_pair_pass<false, true>), dim3(nfull, batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                           w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial, w.pstride,
-                           w.ctl, 0, w.npass);
-      if (nfull < w.npass)
-        hipLaunchKernelGGL((k_pair_pass<false, false>), dim3(w.npass - nfull, batch), dim3(kThreads), 0, s, V, vstride,
-                           n, j, w.w[0], w.w[1], zp, zc, w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, w.partial,
-                           w.pstride, w.ctl, nfull, w.npass);
+      if (nfull > 0) pass(k_pair_pass<false, true>, 0, nfull);
+      if (nfull < w.npass) pass(k_pair_pass<false, false>, nfull, w.npass - nfull);
     }
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pair_reduce, dim3(2 * j + 6, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h,
