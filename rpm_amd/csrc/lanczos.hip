@@ -549,8 +549,10 @@ __device__ inline int pair_mode(const BasisCtl& C, int j) {
 }
 
 // Input of M2: z / s for a pair (M2 scales its input by s, so it forms H z), x_{j+1} for S1.
-// With dots (a single trajectory), every block forms alpha~_j from M1's per-unit partials in the
-// order of k_alpha, and block 0 stores it, in place of a k_alpha launch.
+// kFuse (a single trajectory): every block forms alpha~_j from M1's per-unit partials dots in the
+// order of k_alpha, and block 0 stores it, in place of a k_alpha launch. The batched instantiation
+// carries none of that code.
+template <bool kFuse>
 __global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__ V, long long vstride, long long n,
                                                         int j, const c128* __restrict__ X0, const c128* __restrict__ X1,
                                                         const c128* __restrict__ ZA, c128* __restrict__ Z,
@@ -561,13 +563,11 @@ __global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__
   const int mode = pair_mode(C, j);
   if (mode != kModePair && mode != kModeS1) return;
   double at = 0.0;
-  if (mode == kModePair) {
-    if (dots) {
+  if constexpr (kFuse) {
+    if (mode == kModePair) {
       __shared__ c128 sm[kThreads / kWave];
       at = alpha_sum(dots, units, b, sm);
       if (blockIdx.x == 0 && threadIdx.x == 0) atilde[b] = at;
-    } else {
-      at = atilde[b];
     }
   }
   const long long i = static_cast<long long>(blockIdx.x) * kThreads + threadIdx.x;
@@ -577,6 +577,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_prep(const c128* __restrict__
     Z[o] = X1[o];
     return;
   }
+  if constexpr (!kFuse) at = atilde[b];
   const double s = C.scale, bp = C.beta_prev;
   const c128 xh = cscale(X0[o], s);
   const c128 xp = C.npend == 2 ? cscale(ZA[o], C.zscale) : V[b * vstride + static_cast<long long>(j - 1) * n + i];
@@ -780,7 +781,7 @@ __device__ inline void pair_pass_body(const PairPass& P, c128* s_r, c128* s_q, c
   }
 }
 
-template <bool kStream, bool kFull>
+template <bool kStream, bool kFull, bool kFuse>
 __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, long long vstride, long long n, int j,
                                                         c128* __restrict__ X0, c128* __restrict__ X1,
                                                         const c128* __restrict__ ZA, const c128* __restrict__ Z,
@@ -796,12 +797,13 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
   const BasisCtl& C = ctl[b];
   const int mode = pair_mode(C, j);
   if (mode == kModeIdle) return;
-  // With dots and dots2 (a single trajectory; k_alpha does not run), the alpha~ this pass or the
-  // slot's finish needs comes from the matvec partials, summed in the order of k_alpha: a pair's
+  // kFuse (a single trajectory; k_alpha does not run): the alpha~ this pass or the slot's finish
+  // needs comes from the matvec partials dots, dots2, summed in the order of k_alpha: a pair's
   // alpha~_j was stored by k_pair_prep and its alpha~_{j+1} (finish) is stored by block 0 here; a
-  // single step S0 or S1 forms its own in every block, and block 0 stores it.
-  double at_single = 0.0;   // S0 / S1 with dots: this block's alpha~ (never re-read from memory here)
-  if (dots) {
+  // single step S0 or S1 forms its own in every block, and block 0 stores it. The batched
+  // instantiations carry none of that code.
+  double at_single = 0.0;   // S0 / S1: this block's alpha~ (never re-read from memory here)
+  if constexpr (kFuse) {
     __shared__ c128 s_al[kThreads / kWave];
     const bool lead = blk0 + blockIdx.x == 0;
     if (mode == kModePair) {
@@ -857,7 +859,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
     P.xb = X0 + b * n;
     P.r = X1 + b * n;      // P = H x^_j
     P.y = X1 + b * n;      // y_j, the pending x_{j+1}, over P (each element read before it is written)
-    P.at = dots ? at_single : atilde[b];
+    P.at = kFuse ? at_single : atilde[b];
     if (np2)
       pair_pass_body<kStream, kFull, false, true>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
     else
@@ -867,7 +869,7 @@ __global__ void __launch_bounds__(kThreads) k_pair_pass(c128* __restrict__ V, lo
     P.xb = X1 + b * n;
     P.r = Q + b * n;       // M2 on x_{j+1}
     P.y = X0 + b * n;
-    P.at = dots ? at_single : atilde2[b];
+    P.at = kFuse ? at_single : atilde2[b];
     pair_pass_body<kStream, kFull, false, false>(P, s_r, s_q, s_x, s_a, s_z, s_corr, s_cA, s_cB, s_red);
   }
 }
@@ -1615,8 +1617,12 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
       hipLaunchKernelGGL(k_alpha, dim3(batch), dim3(kThreads), 0, s, w.dots, w.units, w.atilde, j, w.ctl);
       RPM_LAUNCH_CHECK();
     }
-    hipLaunchKernelGGL(k_pair_prep, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0, s, V, vstride, n, j,
-                       w.w[0], w.w[1], zp, zc, w.atilde, fdots, w.units, w.ctl);
+    if (fuse_alpha)
+      hipLaunchKernelGGL(k_pair_prep<true>, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0, s, V, vstride, n,
+                         j, w.w[0], w.w[1], zp, zc, w.atilde, w.dots, w.units, w.ctl);
+    else
+      hipLaunchKernelGGL(k_pair_prep<false>, dim3(ceil_div(n, kThreads), batch), dim3(kThreads), 0, s, V, vstride, n,
+                         j, w.w[0], w.w[1], zp, zc, w.atilde, nullptr, w.units, w.ctl);
     RPM_LAUNCH_CHECK();
     A.j = j + 1;
     A.dots = w.dots2;
@@ -1632,12 +1638,12 @@ int basis_batch(const rpm_operator* op, int batch, const c128* states, long long
                          w.q, w.cpendA, w.cpend, m, w.atilde, w.atilde2, fdots, fdots2, w.units, w.partial, w.pstride,
                          w.ctl, b0, w.npass);
     };
-    if (batch > 1) {
-      if (nfull > 0) pass(k_pair_pass<true, true>, 0, nfull);
-      if (nfull < w.npass) pass(k_pair_pass<true, false>, nfull, w.npass - nfull);
+    if (!fuse_alpha) {
+      if (nfull > 0) pass(k_pair_pass<true, true, false>, 0, nfull);
+      if (nfull < w.npass) pass(k_pair_pass<true, false, false>, nfull, w.npass - nfull);
     } else {
-      if (nfull > 0) pass(k_pair_pass<false, true>, 0, nfull);
-      if (nfull < w.npass) pass(k_pair_pass<false, false>, nfull, w.npass - nfull);
+      if (nfull > 0) pass(k_pair_pass<false, true, true>, 0, nfull);
+      if (nfull < w.npass) pass(k_pair_pass<false, false, true>, nfull, w.npass - nfull);
     }
     RPM_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_pair_reduce, dim3(2 * j + 6, batch), dim3(kThreads), 0, s, w.partial, w.npass, w.pstride, w.h,
