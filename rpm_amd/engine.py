@@ -492,8 +492,8 @@ class BasisGraphs:
             return
         if g is None:
             # capture_begin/capture_end on a side stream rather than the torch.cuda.graph context,
-            # which synchronises, collects garbage and empties the allocator cache at every capture
-            # (6% of a whole-horizon batch); the window allocates nothing, so it needs none of that
+            # which synchronises, collects garbage and empties the allocator cache at every capture;
+            # the window allocates nothing, so it needs none of that
             g = torch.cuda.CUDAGraph()
             cur = torch.cuda.current_stream()
             side = self._side if self._side is not None else torch.cuda.Stream()
