@@ -21,6 +21,7 @@ reference's 0.1 us speed probe, priced on the bytes this implementation moves).
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-extras]
     python bench.py --mode propagate|batch|directional|grad|spectral|sobol|trajectory|copy ...
     python bench.py --mode archive_batch   # 32 whole archived trajectories in one batch (about a minute)
+    python bench.py --mode archive_all     # every archived trajectory (464, about 10 minutes); --archive-groups
     python bench.py --mode gradcheck       # whole-horizon adjoint gradient against central differences
     python bench.py --mode pulse           # one whole-horizon ascent step of the 64-segment pulse
     python bench.py --mode ceiling         # copy and two-pass streaming bounds beside the H·psi kernel
@@ -287,59 +288,93 @@ def trajectory_probe(theta: float = 90.0, quadrupole_scale: float = 1.0, sample:
 
 
 def archive_batch_probe(thetas=(0.0, 20.0, 40.0, 60.0, 80.0, 90.0, 120.0, 160.0), samples: int = 4,
-                        quadrupole_scale: float = 1.0) -> dict:
-    """32 whole trajectories of the reference's archived uncoupled run (8 field angles x samples
-    0-3, D = J = 0, quadrupoles on, the settings of trajectory_probe) in one batched propagation,
-    each yield compared with its archived value. Timed from the host states to the host yields."""
+                        quadrupole_scale: float = 1.0, interactions: str = "D=J=0", batch_size: int | None = None,
+                        label: str = "archive_batch") -> dict:
+    """Whole trajectories of the reference's archived full-system run in batched propagations, each
+    yield compared with its archived value. Default: 32 cells of the uncoupled run (8 field angles x
+    samples 0-3, D = J = 0, quadrupoles on, the settings of trajectory_probe) in one batch. thetas=None
+    takes every archived angle of the (interactions, quadrupole_scale) group; batch_size splits the
+    cells into consecutive batches. Timed from the host states to the host yields."""
     import csv
 
     import numpy as np
     import torch
     from rpm_amd import batch, core
-    model = core.ProductionModel(quadrupole_scale=quadrupole_scale, ratio_D_over_J=None)
+    model = core.ProductionModel(quadrupole_scale=quadrupole_scale,
+                                 ratio_D_over_J=None if interactions == "D=J=0" else -3.0)
+    if interactions not in ("D=J=0", "D/J=-3"):
+        raise ValueError(f"no archived group {interactions!r}")
     cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
                            state_error_budget=2.4e-5, quadrupole_scale=quadrupole_scale)
     archived = {}
     with open(ARCHIVED_YIELDS) as f:
         for r in csv.DictReader(f):
-            if r["electron_interactions"] == "D=J=0" and float(r["scale"]) == quadrupole_scale:
+            if r["electron_interactions"] == interactions and float(r["scale"]) == quadrupole_scale:
                 archived[(float(r["theta_deg"]), int(r["sample_id"]))] = float(r["yield"])
+    if thetas is None:
+        thetas = sorted({t for t, _ in archived})
     cells = [(float(t), s) for t in thetas for s in range(samples)]
     missing = [c for c in cells if c not in archived]
     if missing:
         raise ValueError(f"cells not in the archived run: {missing}")
     times = cfg.time_grid_us
+    size = len(cells) if batch_size is None else max(1, int(batch_size))
 
     def progress(b, now, accepted, rejected, matvecs, acc_err):
         if b == 0 and accepted % 200 == 0:
-            print(f"[bench] archive batch: trajectory 0 at {now:.2f} us, {accepted} steps", file=sys.stderr,
+            print(f"[bench] {label}: trajectory 0 of the batch at {now:.2f} us, {accepted} steps", file=sys.stderr,
                   flush=True)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     base = core.build_hamiltonian(90.0, 0.05, model=model)
-    hs = [core.build_hamiltonian(t, 0.05, model=model) for t in thetas]
-    pairs = batch.pair_matrices(base, [hs[i // samples] for i in range(len(cells))])
-    states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, s) for _, s in cells])
-    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
-    res = bp.propagate(states, times, pairs=pairs, progress_callback=progress,
-                       yield_weights=core.yield_weight_table(times, cfg.lifetime_us, core.coarse_grid_indices(cfg)))
-    yields = np.array([r.device_yields[0] for r in res])       # summed on the device
+    hs = {t: core.build_hamiltonian(t, 0.05, model=model) for t in thetas}
+    weights = core.yield_weight_table(times, cfg.lifetime_us, core.coarse_grid_indices(cfg))
+    yields, host, matvecs_ref, matvecs_dev, rounds = [], [], 0, 0, 0
+    for k in range(0, len(cells), size):
+        part = cells[k:k + size]
+        pairs = batch.pair_matrices(base, [hs[t] for t, _ in part])
+        states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, smp) for _, smp in part])
+        bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
+        res = bp.propagate(states, times, pairs=pairs, progress_callback=progress, yield_weights=weights)
+        yields += [r.device_yields[0] for r in res]       # summed on the device
+        host += [core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res]
+        matvecs_ref += int(sum(r.stats.matvecs for r in res))
+        matvecs_dev += int(bp.device_matvecs.sum())
+        rounds += bp.rounds
+        print(f"[bench] {label}: {min(k + size, len(cells))} of {len(cells)} cells after "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
-    host = np.array([core.integrate_yield(times, r.observable_values, cfg.lifetime_us) for r in res])
+    yields, host = np.array(yields), np.array(host)
     diff = yields - np.array([archived[c] for c in cells])
     horizon = float(times[-1])
     per_traj = dt / len(cells)
     ref = REFERENCE_TRAJECTORY_SECONDS
-    return {"mode": "archive_batch", "trajectories": len(cells), "thetas_deg": list(thetas), "samples": samples,
-            "quadrupole_scale": quadrupole_scale, "horizon_us": horizon, "seconds": dt,
+    return {"mode": label, "electron_interactions": interactions, "quadrupole_scale": quadrupole_scale,
+            "trajectories": len(cells), "thetas_deg": list(thetas), "samples": samples, "batch_size": size,
+            "horizon_us": horizon, "seconds": dt,
             "trajectory_us_per_s": len(cells) * horizon / dt, "seconds_per_trajectory": per_traj,
-            "matvecs_reference_count": int(sum(r.stats.matvecs for r in res)),
-            "matvecs_device": int(bp.device_matvecs.sum()), "rounds": bp.rounds,
+            "matvecs_reference_count": matvecs_ref, "matvecs_device": matvecs_dev, "rounds": rounds,
             "max_abs_yield_minus_archived": float(np.abs(diff).max()),
             "max_abs_device_minus_host_yield": float(np.abs(yields - host).max()),
             "reference_seconds": ref, "speedup_vs_reference_median": ref["median"] / per_traj}
+
+
+def archive_all_probe(groups, batch_size: int) -> dict:
+    """Every archived trajectory of the given (interactions, quadrupole scale) groups (29 angles x 4
+    samples each; the archive holds four groups, 464 trajectories), batched, against its archived yield."""
+    out = {"mode": "archive_all", "groups": []}
+    for interactions, scale in groups:
+        out["groups"].append(archive_batch_probe(thetas=None, quadrupole_scale=scale, interactions=interactions,
+                                                 batch_size=batch_size, label="archive_all"))
+    g = out["groups"]
+    out["trajectories"] = sum(x["trajectories"] for x in g)
+    out["seconds"] = sum(x["seconds"] for x in g)
+    out["seconds_per_trajectory"] = out["seconds"] / out["trajectories"]
+    out["max_abs_yield_minus_archived"] = max(x["max_abs_yield_minus_archived"] for x in g)
+    out["max_abs_device_minus_host_yield"] = max(x["max_abs_device_minus_host_yield"] for x in g)
+    return out
 
 
 def batch_probe(h, args, B=None, horizon=None, plain_sample=None) -> dict:
@@ -717,12 +752,16 @@ def main() -> int:
                     help="hpsi mode: the production operator, or the 14-nucleus random-tensor system")
     ap.add_argument("--horizon", type=float, default=0.05, help="propagate mode: simulated horizon in us")
     ap.add_argument("--plain-sample", type=int, default=8, help="batch mode: plain trajectories timed for the ratio")
+    ap.add_argument("--archive-groups", default="D=J=0:0,D=J=0:1,D/J=-3:0,D/J=-3:1",
+                    help="archive_all mode: comma-separated electron_interactions:quadrupole_scale groups")
+    ap.add_argument("--archive-batch", type=int, default=58,
+                    help="archive_all mode: trajectories per batched propagation (two batches per group of 116)")
     ap.add_argument("--eager", action="store_true",
                     help="batch mode: launch each window of basis steps eagerly instead of replaying it as a "
                          "captured HIP graph (the default)")
     ap.add_argument("--mode", default="hpsi",
                     choices=("hpsi", "copy", "propagate", "batch", "grad", "spectral", "sobol", "directional",
-                             "trajectory", "archive_batch", "gradcheck", "pulse", "ceiling"),
+                             "trajectory", "archive_batch", "archive_all", "gradcheck", "pulse", "ceiling"),
                     help="copy: time a plain device copy of the same bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -764,6 +803,13 @@ def main() -> int:
         return 0
     if args.mode == "archive_batch":
         print(json.dumps(archive_batch_probe()), flush=True)
+        return 0
+    if args.mode == "archive_all":
+        groups = []
+        for item in args.archive_groups.split(","):
+            name, scale = item.rsplit(":", 1)
+            groups.append((name, float(scale)))
+        print(json.dumps(archive_all_probe(groups, args.archive_batch)), flush=True)
         return 0
     if args.mode == "gradcheck":
         print(json.dumps(gradcheck_probe()), flush=True)
