@@ -130,8 +130,9 @@ class BatchedLanczosPropagator:
         # matvecs the device made per trajectory in the last call; stats.matvecs counts the reference's
         # whole-basis calls for the same schedule (bases are built only as far as a candidate needs)
         self.device_matvecs = None
-        # replay each basis step window as a HIP graph (captured once per propagate call); bitwise the
-        # eager launches, and as fast as them once a run has more than a few dozen rounds
+        # replay each basis step window as a HIP graph (captured on the window's third run in a
+        # propagate call); bitwise the eager launches, and as fast as them once a run has more than a
+        # few dozen rounds
         self.use_graphs = True
         self.graphs = None
         # Lanczos steps from the 12th on in pairs that share one read of the basis

@@ -463,8 +463,8 @@ class BasisGraphs:
     """HIP graphs of the basis builder's launch sequences, one per (first step, last step) window.
 
     The builder launches kernels only (no allocation, host synchronisation or memset), so a window
-    is captured (on a side stream, torch.cuda.graph) the ``min_uses``-th time it is run and replayed on
-    the current stream from then on; the per-call targets and stops are read from device memory at
+    is captured (on a side stream, torch.cuda.CUDAGraph capture_begin/capture_end) the ``min_uses``-th
+    time it is run and replayed on the current stream from then on; the per-call targets and stops are read from device memory at
     replay. Earlier runs of a window launch eagerly, so windows that a short propagation meets once
     or twice pay no capture. A cache belongs to one set of buffers: build a new one when the buffers
     change.
