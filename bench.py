@@ -849,7 +849,7 @@ def main() -> int:
     # 50-80 launches run 10-30% slow while the engine clock ramps up (0.70 -> 0.535 ms per launch at
     # B = 256), so a short warm-up would time the ramp, not the kernel.  Launch the same step until
     # 0.2 s of it has run back to back; the count and time are reported.
-    settle_n, settle_ms = 0, 0.0
+    settle_n, settle_ms, settle_groups = 0, 0.0, []
     while settle_ms < 200.0 and settle_n < 4000:
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
@@ -858,7 +858,8 @@ def main() -> int:
         e1.record()
         torch.cuda.synchronize()
         settle_n += 20
-        settle_ms += e0.elapsed_time(e1)
+        settle_groups.append(e0.elapsed_time(e1) / 20)
+        settle_ms += 20 * settle_groups[-1]
     for _ in range(args.warmup):
         h.apply_device(x, y, batch=B, ld=n)
     torch.cuda.synchronize()
@@ -885,6 +886,7 @@ def main() -> int:
         "steps": args.steps,
         "warmup": args.warmup,
         "clock_settle": {"launches": settle_n, "ms": settle_ms,
+                         "ms_per_launch_first_20": settle_groups[0], "ms_per_launch_last_20": settle_groups[-1],
                          "note": "untimed launches of the same step before the warm-up, until 0.2 s has run"},
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
