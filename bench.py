@@ -735,7 +735,7 @@ def ceiling_probe(h, B: int = 256, reps: int = 20) -> dict:
     return out
 
 
-KERNEL_NAMES = {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 9: "k_local_apply"}
+KERNEL_NAMES = {1: "k_pair_apply", 2: "k_pair_tile", 3: "k_pair_tile3", 4: "k_pair_tile5", 9: "k_local_apply"}
 
 
 def main() -> int:

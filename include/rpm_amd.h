@@ -120,9 +120,14 @@ int rpm_operator_apply_pairs(const rpm_operator* op, const void* d_in, void* d_o
  * H·psi kernel selection (no counterpart in the reference; for tests and
  * benchmarks): 0 automatic (the LDS-tiled pair kernel when the shape allows
  * it), 1 the per-amplitude gather kernel, 2 the register-tiled kernel, 3 the
- * LDS-tiled kernel (2-3 give RPM_E_UNSUPPORTED when the operator shape has no
- * tiled form).  rpm_operator_kernel reports the kernel the next apply will
- * use: 1-3 as above, 9 the generic local-term kernel.
+ * LDS-tiled kernel, 4 the LDS-tiled kernel that closes the first left axis
+ * (dimension 3) in its tile instead of the last two (2-4 give
+ * RPM_E_UNSUPPORTED when the operator shape has no such tiled form; the
+ * per-vector Lanczos partials of kernel 4 come one per 3-row group, so a
+ * workspace sized under another kernel may be too small, which the basis
+ * entry points report as RPM_E_INVALID).  rpm_operator_kernel reports the
+ * kernel the next apply will use: 1-4 as above, 9 the generic local-term
+ * kernel.
  */
 int rpm_operator_set_kernel(rpm_operator* op, int kernel);
 int rpm_operator_kernel(const rpm_operator* op);

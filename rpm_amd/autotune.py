@@ -2,7 +2,7 @@
 
 Two choices are tuned by timing real launches on the current stream:
 - ``hpsi_kernel``: which H·psi kernel an operator uses for a batch size (the per-amplitude gather
-  kernel, the register-tiled or the LDS-tiled pair kernel);
+  kernel, the register-tiled pair kernel, or one of the two LDS-tiled pair kernels);
 - ``propagation_batch``: how many trajectories a batched propagation runs per launch, timed on a
   short prefix of the real workload and bounded by device memory.
 
@@ -20,7 +20,7 @@ import numpy as np
 
 from . import _native as nat
 
-HPSI_KERNELS = {1: "gather", 2: "register-tiled", 3: "LDS-tiled"}
+HPSI_KERNELS = {1: "gather", 2: "register-tiled", 3: "LDS-tiled", 4: "LDS-tiled, closed first axis"}
 _CACHE: dict = {}
 _LOCK = threading.Lock()
 

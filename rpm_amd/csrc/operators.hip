@@ -39,7 +39,7 @@ struct PairOp {
   PairTables t;
   TileTables tile;      // valid when use_tile
   bool use_tile;
-  int kernel;           // 0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled
+  int kernel;           // 0 auto, 1 gather, 2 register-tiled, 3 LDS-tiled, 4 LDS-tiled closed first axis
   void* dev;            // single device allocation backing the tables
 };
 
@@ -309,6 +309,63 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
         nzCt |= bits << (4 * (j * 2 + m));
       }
   }
+  // ---- closed-first-axis tile tables (k_pair_tile5): the first left axis (dimension kD0) is closed,
+  // every other left axis is open; group g holds the rows l = g + t * ngroups5
+  bool t5_ok = tile_ok && nL >= 2 && ldims[0] == kD0;
+  int KO5 = 0;
+  for (int i = 1; i < nL && t5_ok; ++i) KO5 += ldims[i] - 1;
+  t5_ok = t5_ok && pair_tile5_supported(static_cast<int>(NR), KO5, KR) && KO5 <= 8;
+  const long long ngroups5 = t5_ok ? NL / kD0 : 0;
+  std::vector<int> ofsO5(ngroups5 * KO5 > 0 ? ngroups5 * KO5 : 1, 0);
+  std::vector<c128> coefO5(ngroups5 * KO5 > 0 ? ngroups5 * KO5 * 4 : 4, c128{0, 0});
+  std::vector<unsigned> nzO5(ngroups5 > 0 ? ngroups5 : 1, 0u), hselO5(ngroups5 > 0 ? ngroups5 : 1, 0u);
+  std::vector<c128> coefF(kD0 * (kD0 - 1) * 4, c128{0, 0});
+  unsigned halfO5 = 0, nzF = 0;
+  if (t5_ok) {
+    std::vector<long long> lstride(nL);
+    {
+      long long st = 1;
+      for (int i = nL - 1; i >= 0; --i) { lstride[i] = st; st *= ldims[i]; }
+    }
+    unsigned both5 = 0;
+    for (long long g = 0; g < ngroups5; ++g) {
+      int k = 0;
+      for (int i = 1; i < nL; ++i) {
+        const int d = ldims[i];
+        const int m = static_cast<int>((g / lstride[i]) % d);
+        const c128* B = blk[i];
+        for (int m2 = 0; m2 < d; ++m2) {
+          if (m2 == m) continue;
+          ofsO5[g * KO5 + k] = static_cast<int>((m2 - m) * lstride[i]);
+          c128* C = &coefO5[(g * KO5 + k) * 4];
+          for (int e = 0; e < 2; ++e)
+            for (int e2 = 0; e2 < 2; ++e2) C[e * 2 + e2] = B[(e * d + m) * (2 * d) + (e2 * d + m2)];
+          unsigned bits = 0;
+          for (int q = 0; q < 4; ++q) bits |= nz(C[q]) ? (1u << q) : 0u;
+          nzO5[g] |= bits << (4 * k);
+          if ((bits & 10u) != 0 && (bits & 5u) == 0) hselO5[g] |= 1u << k;
+          if ((bits & 5u) != 0 && (bits & 10u) != 0) both5 |= 1u << k;
+          ++k;
+        }
+      }
+    }
+    for (int k = 0; k < KO5; ++k)
+      if (!((both5 >> k) & 1u)) halfO5 |= 1u << k;
+    const c128* B = blk[0];      // (2 kD0) x (2 kD0), row = e * kD0 + t
+    for (int t = 0; t < kD0; ++t) {
+      int sidx = 0;
+      for (int t2 = 0; t2 < kD0; ++t2) {
+        if (t2 == t) continue;
+        c128* C = &coefF[(t * (kD0 - 1) + sidx) * 4];
+        for (int e = 0; e < 2; ++e)
+          for (int e2 = 0; e2 < 2; ++e2) C[e * 2 + e2] = B[(e * kD0 + t) * (2 * kD0) + (e2 * kD0 + t2)];
+        unsigned bits = 0;
+        for (int q = 0; q < 4; ++q) bits |= nz(C[q]) ? (1u << q) : 0u;
+        nzF |= bits << (4 * (t * (kD0 - 1) + sidx));
+        ++sidx;
+      }
+    }
+  }
   PairOp* op = new PairOp();
   PairTables& T = op->t;
   T.NL = static_cast<int>(NL);
@@ -327,7 +384,15 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
   const size_t o_oO = reserve(ofsO.size() * 4), o_cO = reserve(coefO.size() * 16), o_fO = reserve(maskO.size() * 4);
   const size_t o_pR = reserve(packR.size() * 4), o_cT = reserve(coefT.size() * 16);
   const size_t o_nzO = reserve(nzO.size() * 4), o_hsO = reserve(hselO.size() * 4);
+  const size_t o_oO5 = reserve(ofsO5.size() * 4), o_cO5 = reserve(coefO5.size() * 16);
+  const size_t o_nzO5 = reserve(nzO5.size() * 4), o_hsO5 = reserve(hselO5.size() * 4);
+  const size_t o_cF = reserve(coefF.size() * 16);
   std::vector<char> host(bytes);
+  std::memcpy(host.data() + o_oO5, ofsO5.data(), ofsO5.size() * 4);
+  std::memcpy(host.data() + o_cO5, coefO5.data(), coefO5.size() * 16);
+  std::memcpy(host.data() + o_nzO5, nzO5.data(), nzO5.size() * 4);
+  std::memcpy(host.data() + o_hsO5, hselO5.data(), hselO5.size() * 4);
+  std::memcpy(host.data() + o_cF, coefF.data(), coefF.size() * 16);
   std::memcpy(host.data() + o_cC, coefC.data(), coefC.size() * 16);
   std::memcpy(host.data() + o_oO, ofsO.data(), ofsO.size() * 4);
   std::memcpy(host.data() + o_cO, coefO.data(), coefO.size() * 16);
@@ -397,6 +462,16 @@ static int build_pair(int nL, const int* ldims, int nR, const int* rdims, const 
     U.hselO = reinterpret_cast<const unsigned*>(base + o_hsO);
     U.halfO = halfO;
     U.nzC = nzCt;
+    U.t5 = t5_ok;
+    U.ngroups5 = static_cast<int>(ngroups5);
+    U.KO5 = KO5;
+    U.ofsO5 = reinterpret_cast<const int*>(base + o_oO5);
+    U.coefO5 = reinterpret_cast<const c128*>(base + o_cO5);
+    U.nzO5 = reinterpret_cast<const unsigned*>(base + o_nzO5);
+    U.hselO5 = reinterpret_cast<const unsigned*>(base + o_hsO5);
+    U.halfO5 = halfO5;
+    U.coefF = reinterpret_cast<const c128*>(base + o_cF);
+    U.nzF = nzF;
   }
   op->dev = dev;
   *out = op;
@@ -555,6 +630,7 @@ static int build_local(int naxes, const int* dims, int nterms, const int* tnaxes
 int apply_units(const rpm_operator* op) {
   if (op->kind == 1) {
     const PairOp* P = static_cast<const PairOp*>(op->impl);
+    if (P->use_tile && P->kernel == 4) return P->tile.ngroups5;
     if (P->use_tile && P->kernel != 1) return P->tile.ngroups;
     return ceil_div(P->t.NLR, 256);
   }
@@ -572,9 +648,11 @@ int operator_apply_ex(const rpm_operator* op, const c128* in, c128* out, int bat
   if (op->kind == 1) {
     const PairOp* P = static_cast<const PairOp*>(op->impl);
     if (P->use_tile && P->kernel != 1) {
-      if (static_cast<long long>(P->tile.ngroups) * batch > 0x7fffffffLL)
+      if (static_cast<long long>(P->tile.ngroups5 > P->tile.ngroups ? P->tile.ngroups5 : P->tile.ngroups) * batch >
+          0x7fffffffLL)
         return fail(RPM_E_INVALID, "batch %d too large for one launch", batch);
       if (P->kernel == 2) return launch_pair_tile(P->tile, in, out, batch, ld, A, stream);
+      if (P->kernel == 4) return launch_pair_tile5(P->tile, in, out, batch, ld, A, stream);
       return launch_pair_tile3(P->tile, in, out, batch, ld, A, stream);
     }
     dim3 grid(ceil_div(P->t.NLR, 256), batch);
@@ -682,10 +760,12 @@ extern "C" int rpm_operator_apply_affine(const rpm_operator* op, const void* d_i
 }
 
 extern "C" int rpm_operator_set_kernel(rpm_operator* op, int kernel) {
-  if (!op || kernel < 0 || kernel > 3) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
+  if (!op || kernel < 0 || kernel > 4) return fail(RPM_E_INVALID, "rpm_operator_set_kernel: bad arguments");
   if (op->kind != 1) return kernel >= 2 ? fail(RPM_E_UNSUPPORTED, "tiled kernels need a pair operator") : RPM_OK;
   PairOp* P = static_cast<PairOp*>(op->impl);
   if (kernel >= 2 && !P->use_tile) return fail(RPM_E_UNSUPPORTED, "the operator shape has no tiled kernel");
+  if (kernel == 4 && !P->tile.t5)
+    return fail(RPM_E_UNSUPPORTED, "the operator shape has no closed-first-axis tiled kernel");
   P->kernel = kernel;
   return RPM_OK;
 }
@@ -695,5 +775,6 @@ extern "C" int rpm_operator_kernel(const rpm_operator* op) {
   if (op->kind != 1) return 9;
   const PairOp* P = static_cast<const PairOp*>(op->impl);
   if (!P->use_tile || P->kernel == 1) return 1;
+  if (P->kernel == 4) return 4;
   return P->kernel == 2 ? 2 : 3;
 }

@@ -414,6 +414,167 @@ k_pair_tile3(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, 
   }
 }
 
+
+// Closed-first-axis variant: one workgroup holds the kD0 = 3 rows l = g + t * ngroups5 of the first
+// left axis (the 14N5 of the production layout) over all NR right indices, 36 KB of amplitudes, so
+// that axis's two neighbour slabs are read from LDS; every other left axis is open (L2), its
+// offsets and coefficients uniform over the tile. Against k_pair_tile3 it loads one amplitude
+// fewer per output (two full 14N5 slabs off, the H8c and Hbeta half slots on) and its 576-thread,
+// 50 KB workgroups fit three to a CU. One (t, r) per thread, 4 outputs per thread.
+template <int NR, int KO, int KR, unsigned HALF>
+__global__ void __launch_bounds__(kD0 * NR) __attribute__((amdgpu_waves_per_eu(7, 8)))
+k_pair_tile5(TileTables T, const c128* __restrict__ in, c128* __restrict__ out, long long ld, int batch, ApplyCtl A) {
+  constexpr int NT = kD0 * NR;
+  static_assert(NR % kWave == 0, "a wave must hold a single left index");
+  static_assert(KR % 4 == 0, "right neighbour descriptors are fetched four at a time");
+  __shared__ c128 tile[4 * kD0 * NR];
+  __shared__ c128 coefT[KR * 3 * 4];
+  __shared__ c128 tabB[4 * NR];   // dB[0], dB[1], fB[0], fB[1] over r
+  int b, grp;
+  tile_map(T.ngroups5, batch, b, grp);
+  double s;
+  if (!tile_begin(A, b, s)) return;
+  const int tid = threadIdx.x;
+  const int t = __builtin_amdgcn_readfirstlane(tid / NR);     // first-axis digit, wave-uniform
+  const int r = tid - t * NR;
+  const int l = grp + t * T.ngroups5;
+  const long long S = T.NLR;
+  const c128* __restrict__ x = in + b * ld;
+  const long long own = static_cast<long long>(l) * NR + r;
+  const long long Sb = S * static_cast<long long>(sizeof(c128));
+  const __amdgpu_buffer_rsrc_t xr = vector_rsrc(x, 4 * Sb);
+  const unsigned ob = static_cast<unsigned>(own * static_cast<long long>(sizeof(c128)));
+
+  for (int i = tid; i < KR * 3 * 4; i += NT) coefT[i] = T.coefT[i];
+  for (int i = tid; i < 4 * NR; i += NT) {
+    const int q = i / NR;
+    tabB[i] = (q < 2 ? T.dB : T.fB)[(q & 1) * NR + (i - q * NR)];
+  }
+  {
+    c128 psi[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      psi[e] = bld16(xr, ob, static_cast<unsigned>(e * Sb));
+      tile[(e * kD0 + t) * NR + r] = psi[e];
+    }
+  }
+
+  // open left axes (global / L2): offsets and coefficients uniform over the tile
+  c128 acc[4];
+#pragma unroll
+  for (int e = 0; e < 4; ++e) acc[e] = cmake(0.0, 0.0);
+  {
+    const int* __restrict__ ofs = T.ofsO5 + grp * KO;
+    kconst<c128>* cof = kc(T.coefO5 + static_cast<long long>(grp) * KO * 4);
+    const unsigned nzo = T.nzO5[grp];
+    const unsigned hs = T.hselO5[grp];
+#pragma unroll
+    for (int k = 0; k < KO; ++k) {
+      const unsigned o = ob + static_cast<unsigned>(ofs[k] * NR * static_cast<int>(sizeof(c128)));
+      const unsigned nzk = (nzo >> (4 * k)) & 0xfu;
+      c128 q[4];
+      if ((HALF >> k) & 1u) {
+        const unsigned h = 2 * ((hs >> k) & 1u);
+        q[0] = bld16(xr, o, static_cast<unsigned>(h * Sb));
+        q[1] = bld16(xr, o, static_cast<unsigned>((h + 1) * Sb));
+        mac_left_nz(cof + k * 4, nzk, &q[0], &q[0], acc);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) q[e] = bld16(xr, o, static_cast<unsigned>(e * Sb));
+        mac_left_nz(cof + k * 4, nzk, &q[0], &q[2], acc);
+      }
+    }
+  }
+
+  __syncthreads();
+  {
+    c128 psi[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) psi[e] = tile[(e * kD0 + t) * NR + r];
+    PairCoef E;
+    load_pair_coef(A.pairs ? A.pairs + b * 16 : T.pair, E);
+    const c128 dA0 = T.dA[l], dA1 = T.dA[T.NL + l];
+    const c128 fA0 = T.fA[l], fA1 = T.fA[T.NL + l];
+    const c128 dB0 = tabB[r], dB1 = tabB[NR + r], fB0 = tabB[2 * NR + r], fB1 = tabB[3 * NR + r];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const c128 d = cadd(cadd((e >> 1) ? dA1 : dA0, (e & 1) ? dB1 : dB0), E.d[e]);
+      c128 a = cfma(d, psi[e], acc[e]);
+      a = cfma(cadd((e >> 1) ? fA1 : fA0, E.fa[e]), psi[e ^ 2], a);
+      a = cfma(cadd((e & 1) ? fB1 : fB0, E.fb[e]), psi[e ^ 1], a);
+      a = cfma(E.bo[e], psi[e ^ 3], a);
+      acc[e] = a;
+    }
+  }
+  // the closed first axis: the other two rows of the tile (t uniform over a wave)
+#pragma unroll
+  for (int sidx = 0; sidx < kD0 - 1; ++sidx) {
+    const int t2 = sidx < t ? sidx : sidx + 1;
+    const unsigned nzf = (T.nzF >> (4 * (t * (kD0 - 1) + sidx))) & 0xfu;
+    kconst<c128>* C = kc(T.coefF) + (t * (kD0 - 1) + sidx) * 4;
+    c128 q[4];
+    if ((nzf & 5u) == 0 || (nzf & 10u) == 0) {
+      const int h = (nzf & 10u) ? 2 : 0;
+      q[0] = tile[(h * kD0 + t2) * NR + r];
+      q[1] = tile[((h + 1) * kD0 + t2) * NR + r];
+      mac_left_nz(C, nzf, &q[0], &q[0], acc);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) q[e] = tile[(e * kD0 + t2) * NR + r];
+      mac_left_nz(C, nzf, &q[0], &q[2], acc);
+    }
+  }
+  int pkr[KR];
+#pragma unroll
+  for (int k4 = 0; k4 < KR / 4; ++k4) {
+    const int4 v = reinterpret_cast<const int4*>(T.packR + r * KR)[k4];
+    pkr[4 * k4] = v.x;
+    pkr[4 * k4 + 1] = v.y;
+    pkr[4 * k4 + 2] = v.z;
+    pkr[4 * k4 + 3] = v.w;
+  }
+#pragma unroll
+  for (int k = 0; k < KR; ++k) {
+    const int pk = pkr[k];
+    const int off = static_cast<short>(pk & 0xffff);
+    const int m = pk >> 16;
+    c128 C[4], q[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) C[i] = coefT[(k * 3 + m) * 4 + i];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) q[e] = tile[(e * kD0 + t) * NR + r + off];
+    mac2x2_right(C, (T.maskR >> (2 * k)) & 3, q, acc);
+  }
+
+  char* __restrict__ yb = reinterpret_cast<char*>(out + b * ld);
+  const double bprev = A.prev ? A.ctl[b].beta_prev : 0.0;
+  const bool plain = !A.ctl && !A.affine;   // s == 1: the output is H in itself
+  c128 dot{0.0, 0.0};
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const c128 pe = tile[(e * kD0 + t) * NR + r];
+    c128 v = plain ? acc[e] : apply_epilogue(A, acc[e], pe, s, b, ld, e * S + own);
+    if (A.dots) dot = cfmaconj(cscale(pe, s), v, dot);
+    if (A.prev) {
+      const c128 p = A.prev[b * A.prev_ld + e * S + own];
+      v.re = fma(-bprev, p.re, v.re);
+      v.im = fma(-bprev, p.im, v.im);
+    }
+    c128* yo = reinterpret_cast<c128*>(yb + e * Sb + ob);
+    __builtin_nontemporal_store(v.re, &yo->re);
+    __builtin_nontemporal_store(v.im, &yo->im);
+  }
+  if (A.copy) {
+    c128* z = A.copy + b * A.copy_ld;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) z[e * S + own] = cscale(tile[(e * kD0 + t) * NR + r], s);
+  }
+  if (A.dots) {
+    __shared__ c128 red[NT / kWave];
+    block_dot_store(dot, red, A.dots + b * A.dots_ld + grp);
+  }
+}
+
 bool pair_tile_supported(int NR, int NC, int KO, int KR) { return NR == 192 && NC == 2 && KO == 7 && KR == 8; }
 
 int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
@@ -441,6 +602,26 @@ int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch,
                        out, ld, batch, A);
   else
     hipLaunchKernelGGL((k_pair_tile3<192, 2, 7, 8, 0u>), dim3(T.ngroups * batch), dim3(192 * 4), 0, stream, T, in,
+                       out, ld, batch, A);
+  RPM_LAUNCH_CHECK();
+  return RPM_OK;
+}
+
+bool pair_tile5_supported(int NR, int KO5, int KR) { return NR == 192 && KO5 == 7 && KR == 8; }
+
+int launch_pair_tile5(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                      hipStream_t stream) {
+  if (!T.t5 || !pair_tile5_supported(T.NR, T.KO5, T.KR))
+    return fail(RPM_E_UNSUPPORTED, "no closed-first-axis tiled kernel for NR=%d KO=%d KR=%d", T.NR, T.KO5, T.KR);
+  if (4 * T.NLR * static_cast<long long>(sizeof(c128)) > 0x7fffffffLL)
+    return fail(RPM_E_UNSUPPORTED, "vector of %lld amplitudes too large for the tiled kernel", 4 * T.NLR);
+  // open slots 3-6 (the four open spin-1/2 protons H8a, H8b, H8c, Hbeta of the production layout) read one half
+  constexpr unsigned kHalf5 = 0x78u;
+  if ((T.halfO5 & kHalf5) == kHalf5)
+    hipLaunchKernelGGL((k_pair_tile5<192, 7, 8, kHalf5>), dim3(T.ngroups5 * batch), dim3(192 * kD0), 0, stream, T, in,
+                       out, ld, batch, A);
+  else
+    hipLaunchKernelGGL((k_pair_tile5<192, 7, 8, 0u>), dim3(T.ngroups5 * batch), dim3(192 * kD0), 0, stream, T, in,
                        out, ld, batch, A);
   RPM_LAUNCH_CHECK();
   return RPM_OK;

@@ -33,12 +33,29 @@ struct TileTables {
   const unsigned* hselO;           // [ngroups] bit k: the input half a half slot k reads
   unsigned halfO;                  // bit k: open slot k is a half slot in every group
   unsigned nzC;                    // one nibble per closed (axis j, digit m) at index j * 2 + m
+  // Closed-first-axis tile (k_pair_tile5): the first left axis (dimension kD0 = 3, the 14N5 of the
+  // production layout) is closed in the tile and every other left axis is open. Group g holds the
+  // rows l = g + t * ngroups5 (t < kD0) over all NR right indices.
+  bool t5;                         // the tables below are valid
+  int ngroups5, KO5;
+  const int* ofsO5;                // [ngroups5][KO5]  left-index offset of each open neighbour
+  const c128* coefO5;              // [ngroups5][KO5][4]
+  const unsigned* nzO5;            // [ngroups5] one nibble per open slot
+  const unsigned* hselO5;          // [ngroups5] bit k: the input half a half slot k reads
+  unsigned halfO5;                 // bit k: open slot k is a half slot in every group
+  const c128* coefF;               // [kD0][kD0 - 1][4]  first-axis block of digit t to its s-th other digit
+  unsigned nzF;                    // one nibble per (t, s) at index t * (kD0 - 1) + s
 };
+
+constexpr int kD0 = 3;             // dimension of the closed first left axis of k_pair_tile5
 
 int launch_pair_tile(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
                      hipStream_t stream);
 int launch_pair_tile3(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
                       hipStream_t stream);
+int launch_pair_tile5(const TileTables& T, const c128* in, c128* out, int batch, long long ld, const ApplyCtl& A,
+                      hipStream_t stream);
 bool pair_tile_supported(int NR, int NC, int KO, int KR);
+bool pair_tile5_supported(int NR, int KO5, int KR);
 
 }  // namespace rpm

@@ -88,7 +88,8 @@ class _DeviceOperator:
         self.handle = 0
 
     def set_kernel(self, kernel: int) -> None:
-        """0 automatic, 1 per-amplitude gather kernel, 2 register-tiled, 3 LDS-tiled pair kernel."""
+        """0 automatic, 1 per-amplitude gather kernel, 2 register-tiled, 3 LDS-tiled pair kernel, 4 LDS-tiled
+        with the first left axis (dimension 3) closed in the tile."""
         nat.check(nat.load().rpm_operator_set_kernel(ctypes.c_void_p(self.handle), int(kernel)))
 
     @property

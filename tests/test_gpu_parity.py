@@ -388,7 +388,7 @@ def test_random_tensor_full_system_propagation_matches_oracle():
     assert _rel(a.final_state, b.final_state) < 1e-10
 
 
-@pytest.mark.parametrize("kernel", [2, 3])
+@pytest.mark.parametrize("kernel", [2, 3, 4])
 @pytest.mark.parametrize("batch", [1, 3, 8, 16])
 def test_tiled_kernel_matches_gather_kernel_and_oracle(production_pair, batch, kernel):
     h, o = production_pair
