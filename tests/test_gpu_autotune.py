@@ -28,7 +28,7 @@ def test_hpsi_kernel_tuning_production():
     op = h._device_op()
     tuned = autotune.hpsi_kernel(h, 8)
     print(tuned, flush=True)
-    assert set(tuned["ms_per_launch"]) == {1, 2, 3}
+    assert set(tuned["ms_per_launch"]) == {1, 2, 3, 4}
     assert tuned["kernel"] == min(tuned["ms_per_launch"], key=tuned["ms_per_launch"].get)
     assert op.kernel == tuned["kernel"]
     assert autotune.hpsi_kernel(h, 7) is tuned            # same power-of-two bucket: cached
