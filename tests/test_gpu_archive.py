@@ -85,3 +85,30 @@ def test_archived_yields_batched():
         got = core.integrate_yield(times, r.observable_values, cfg.lifetime_us)
         print(f"theta={t}: batched {got:.15f} archive {ref:.15f} diff {got - ref:.2e}", flush=True)
         assert abs(got - ref) < 1e-10
+
+
+def test_archived_coupled_yields_batched():
+    """Four archived cells of the coupled run (D/J = -3, quadrupoles off, four field angles, samples
+    0-3) in one batched propagation over the whole horizon, the device-summed yields against the
+    archive to 1e-10."""
+    from rpm_amd import batch
+    rows = _rows()
+    cells = ((10.0, 0), (85.0, 1), (91.0, 2), (150.0, 3))
+    print(f"case: batched archived coupled cells {cells}", flush=True)
+    model = core.ProductionModel(quadrupole_scale=0.0, ratio_D_over_J=-3.0)
+    cfg = core.SweepConfig(seed=core.MASTER_SEED, field_microT=50.0, lifetime_us=5.0, survival_tail_bound=1e-5,
+                           state_error_budget=2.4e-5, quadrupole_scale=0.0)
+    base = core.build_hamiltonian(90.0, 0.05, model=model)
+    pairs = batch.pair_matrices(base, [core.build_hamiltonian(t, 0.05, model=model) for t, _ in cells])
+    states = np.stack([core.random_phase_initial_state(base.dimensions, core.MASTER_SEED, s) for _, s in cells])
+    for (t, s), st in zip(cells, states):
+        assert core.initial_state_sha256(st) == _pick(rows, "D/J=-3", 0.0, t, s)["initial_state_sha256"]
+    bp = batch.BatchedLanczosPropagator.like(core.make_propagator(base, cfg))
+    times = cfg.time_grid_us
+    weights = core.yield_weight_table(times, cfg.lifetime_us, core.coarse_grid_indices(cfg))
+    res = bp.propagate(states, times, pairs=pairs, yield_weights=weights)
+    for (t, s), r in zip(cells, res):
+        ref = float(_pick(rows, "D/J=-3", 0.0, t, s)["yield"])
+        got = float(r.device_yields[0])
+        print(f"theta={t} sample={s}: batched {got:.15f} archive {ref:.15f} diff {got - ref:.2e}", flush=True)
+        assert abs(got - ref) < 1e-10
